@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark: the plain single-domain pipeline keys -> sort -> tree -> neighbors on one MI355X.
+
+Workload (16,777,216 fp32 particles): Plummer sphere (a = 0.05, truncated at r = 1) in the box [-1,1]^3, h from
+the analytic Plummer density refined by 3 iterations of h <- h (100/n_i)^(1/3) on GPU neighbor counts
+(100 +- 10 neighbors), 64-bit Hilbert keys, bucketSize 64, ngmax 150. One step = displace every particle by
+1e-3 h (uniform in [-1,1)^3) + keys + stable sort/gather + one warm-start updateOctree + link + centers + layout +
+findNeighbors. Inputs stay resident in HBM; W warm-up steps, then K timed steps.
+
+Prints ONE JSON line (rank 0). Extra fields: `roofline` for the dominant kernel (the neighbor search), `stages`
+(per-stage HIP-event times over the timed steps), `latency_ms` (p50/p99 per step) and `cpu_baseline` (the
+reference's CPU path, compiled from its own headers into oracle/_ref, timed on this host on a bounded sample).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cornerstone-octree_amd"))
+
+METRIC = "Mparticles/s keys→sort→tree→neighbors (16M fp32), batched vs plain; p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def prepare_h(hip, pipe, x, y, z, h, n, box, target=100.0, iters=3):
+    """Analytic initial h, then `iters` updates h <- h (target/n_i)^(1/3) from GPU neighbor counts.
+    Returns host arrays in SFC order."""
+    nc = hip.DeviceArray(n, np.uint32)
+    d = [hip.DeviceArray.from_numpy(a) for a in (x, y, z, h)]
+    for it in range(iters):
+        pipe.sync(*d, n, box, full_build=True, neighbors=None, counts=nc)
+        a = pipe.arrays()
+        cnt = nc.download().astype(np.float64)
+        hs = a["h"].astype(np.float64) * np.cbrt(target / np.maximum(cnt, 1.0))
+        x, y, z, h = a["x"], a["y"], a["z"], hs.astype(np.float32)
+        log(f"  h iteration {it}: neighbors mean {cnt.mean():.1f} p1 {np.percentile(cnt, 1):.0f} "
+            f"p99 {np.percentile(cnt, 99):.0f}")
+        for dv, hv in zip(d, (x, y, z, h)):
+            dv.upload(hv)
+    pipe.sync(*d, n, box, full_build=True, neighbors=None, counts=nc)
+    cnt = nc.download()
+    a = pipe.arrays()
+    return a["x"], a["y"], a["z"], a["h"], cnt
+
+
+def cpu_baseline(x, y, z, h, stride, seed):
+    """Reference CPU path (oracle/_ref) on the same particles: one full build, then one displaced warm step timed,
+    neighbors of every `stride`-th particle; the neighbor time is scaled by `stride`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_py import REF, make_box
+
+    if REF is None:
+        return None
+    box = make_box(-1.0, 1.0, np.float32)
+    x, y, z, h = (np.array(a, dtype=np.float32, copy=True) for a in (x, y, z, h))
+    tree, counts, t0, _ = REF.pipeline(x, y, z, h, box, 64, 150, nb_stride=stride)
+    rng = np.random.default_rng(seed)
+    disp = (rng.random((3, x.size)) * 2 - 1).astype(np.float32) * (np.float32(1e-3) * h)
+    x = np.clip(x + disp[0], -1, 1).astype(np.float32)
+    y = np.clip(y + disp[1], -1, 1).astype(np.float32)
+    z = np.clip(z + disp[2], -1, 1).astype(np.float32)
+    tree, counts, t, nc = REF.pipeline(x, y, z, h, box, 64, 150, nb_stride=stride, tree=tree, counts=counts)
+    total = t[0] + t[1] + t[2] + t[3] + t[4] + t[5] * stride
+    return {
+        "value": x.size / (total / 1e3) / 1e6,
+        "unit": "Mparticles/s",
+        "cores": REF.omp_threads(),
+        "kind": "reference",
+        "sample": (f"one warm step of the same {x.size}-particle workload through the reference CPU OpenMP path "
+                   f"(oracle/_ref); keys/sort/tree/link/centers on all particles ({sum(t[:5]):.0f} ms), neighbors for "
+                   f"every {stride}th particle ({t[5]:.0f} ms, x{stride}); mean neighbors {nc.mean():.1f}"),
+        "stage_ms": {"keys": t[0], "sort_gather": t[1], "tree_update": t[2], "link": t[3], "centers_layout": t[4],
+                     "neighbors_extrapolated": t[5] * stride},
+    }
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "neighbors_c2_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=16777216)
+    ap.add_argument("--ngmax", type=int, default=150)
+    ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
+              "of the pipeline is a design in DESIGN.md, not a runnable path", file=sys.stderr)
+        sys.exit(2)
+
+    import hipcstone as hip
+    from hipcstone import inputs
+
+    n, K, W, ngmax = args.n, args.steps, args.warmup, args.ngmax
+    T = np.float32
+    a = 0.05
+    t0 = time.time()
+    x, y, z, r = inputs.plummer(n, a, seed=2024, dtype=T)
+    h = inputs.plummer_h(r, n, a, 100.0, dtype=T)
+    log(f"input: {n} Plummer particles ({time.time() - t0:.1f} s)")
+    box = hip.make_box(-1.0, 1.0, T)
+
+    pipe = hip.Pipeline(curve=hip.HILBERT, key_bits=64, real_bits=32, bucket_size=64, ngmax=ngmax)
+    x, y, z, h, cnt0 = prepare_h(hip, pipe, x, y, z, h, n, box)
+    host_particles = (x.copy(), y.copy(), z.copy(), h.copy())
+    log(f"h ready: neighbors mean {cnt0.mean():.1f}, within 100+-10: {np.mean(np.abs(cnt0 - 100.0) <= 10):.3f}")
+
+    nb = hip.DeviceArray(n * ngmax, np.uint32)
+    nc = hip.DeviceArray(n, np.uint32)
+    d = [hip.DeviceArray.from_numpy(v) for v in (x, y, z, h)]
+    pipe.sync(*d, n, box, full_build=True, neighbors=nb, counts=nc)  # converged start tree
+
+    def step(k):
+        v = pipe.view()
+        hip.displace(v.x, v.y, v.z, v.h, n, 1e-3, 7, k, box)
+        pipe.sync(v.x, v.y, v.z, v.h, n, box, full_build=False, neighbors=nb, counts=nc)
+
+    for k in range(W):
+        step(k)
+    hip.device_sync()
+    log(f"warm-up done ({W} steps)")
+
+    events = [hip.Event() for _ in range(K + 1)]
+    pipe.set_timing(True)
+    hip.device_sync()
+    events[0].record()
+    for k in range(K):
+        step(W + k)
+        events[k + 1].record()
+    hip.device_sync()
+    per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
+    total_ms = events[0].elapsed_ms(events[K])
+    stages = pipe.stage_times()
+    pipe.set_timing(False)
+
+    counts = nc.download()
+    kbar = float(np.minimum(counts, ngmax).mean())
+    v = pipe.view()
+    ms_per_step = total_ms / K
+    value = n * K / (total_ms / 1e3) / 1e6
+
+    nb_ms, nb_launch = stages["neighbors"]
+    nb_avg = nb_ms / max(nb_launch, 1)
+    nb_bytes = n * (16 + 4) + 4.0 * float(np.minimum(counts, ngmax).sum())
+    achieved = nb_bytes / (nb_avg / 1e3) / 1e9
+    traffic = load_traffic()
+
+    # whole-step algorithmic bytes (per particle): displace 16 read + 12 write, keys 12+8, sort 8 read + 8+4 write,
+    # gather 4+16+16, neighbors 16+4+4k
+    step_bytes = n * (28 + 20 + 20 + 36 + 20 + 4.0 * kbar)
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("plain single-GPU pipeline: 16,777,216 fp32 Plummer particles (a=0.05, r<=1, box [-1,1]^3), "
+                         "Hilbert-64 keys, bucketSize 64, warm-start updateOctree per step after a 1e-3 h "
+                         "displacement, findNeighbors ngmax 150"),
+            "particles": n, "bucket_size": 64, "ngmax": ngmax, "keys": "hilbert-64",
+            "leaves": int(v.num_leaves), "nodes": int(v.num_nodes), "mean_neighbors": round(float(counts.mean()), 2),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
+                       "p99": round(float(np.percentile(per_step, 99)), 3),
+                       "max": round(float(per_step.max()), 3)},
+        "roofline": {
+            "kernel": "findNeighborsKernel",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "avg_launch_ms": round(nb_avg, 3),
+            "algorithmic_bytes_per_launch": nb_bytes,
+        },
+        "pipeline_roofline": {
+            "algorithmic_bytes_per_step": step_bytes,
+            "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
+            "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()},
+    }
+    if not args.no_cpu_baseline:
+        t1 = time.time()
+        cb = cpu_baseline(*host_particles, args.cpu_stride, seed=11)
+        if cb is not None:
+            log(f"cpu baseline: {time.time() - t1:.1f} s")
+        result["cpu_baseline"] = cb
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

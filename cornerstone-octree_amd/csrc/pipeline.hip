@@ -1,0 +1,513 @@
+// Host orchestration of the plain single-domain pipeline (C++ above the kernels):
+//   keys -> stable sort + gather x,y,z,h -> updateOctree (computeOctree on the first call) -> link ->
+//   geometric centers -> leaf layout -> findNeighbors.
+//
+// This is the sequence of the reference's CPU pipeline (test/unit/neighbors/findneighbors.cpp:43-93,
+// test/performance/octree.cpp:29-100) and of one Domain::sync at numRanks=1 restricted to the global tree
+// (domain/domain.hpp:169-218, assignment.hpp:92-144): the first call converges the tree from the root
+// (computeOctree, tree/csarray.hpp:429-440), later calls apply one warm-start updateOctree step to the previous
+// leaves and counts (csarray.hpp:408-426; performance/octree.cpp:43-44).
+//
+// All buffers are allocated at creation or when a capacity is exceeded (outside steady-state steps). A step
+// synchronizes the stream once per updateOctree call (the new leaf count decides launch sizes) and once to read
+// the level range.
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "runtime.hpp"
+#include "scan.hpp"
+
+
+namespace hipcstone
+{
+
+struct DeviceBuf
+{
+    void* ptr    = nullptr;
+    size_t bytes = 0;
+    ~DeviceBuf()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+    hipcstone_status reserve(size_t b)
+    {
+        if (b <= bytes) return HIPCSTONE_OK;
+        if (ptr) HC_CHECK(hipFree(ptr));
+        ptr   = nullptr;
+        bytes = 0;
+        size_t grown = b + b / 8 + 256;
+        hipError_t e = hipMalloc(&ptr, grown);
+        if (e != hipSuccess)
+        {
+            ptr = nullptr;
+            setError(std::string("pipeline: hipMalloc failed: ") + hipGetErrorString(e));
+            return HIPCSTONE_OUT_OF_MEMORY;
+        }
+        bytes = grown;
+        return HIPCSTONE_OK;
+    }
+    template<class V>
+    V* as() const
+    {
+        return static_cast<V*>(ptr);
+    }
+};
+
+// HIP events around each stage of a sync step, read back lazily (no synchronization while recording)
+class StageTimer
+{
+public:
+    static constexpr int kStages = HIPCSTONE_PIPELINE_STAGES;
+    ~StageTimer()
+    {
+        for (auto& r : pending_)
+        {
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
+        }
+        for (auto e : pool_)
+            (void)hipEventDestroy(e);
+    }
+    void enable(bool on)
+    {
+        on_ = on;
+        for (int i = 0; i < kStages; ++i)
+        {
+            ms_[i]  = 0;
+            cnt_[i] = 0;
+        }
+    }
+    hipcstone_status begin(int stage, hipcstone_stream s)
+    {
+        if (!on_) return HIPCSTONE_OK;
+        Rec r{stage, get(), get()};
+        if (!r.start || !r.stop) return invalid("stage timer: event creation failed");
+        HC_CHECK(hipEventRecord(r.start, stream(s)));
+        pending_.push_back(r);
+        return HIPCSTONE_OK;
+    }
+    hipcstone_status end(hipcstone_stream s)
+    {
+        if (!on_ || pending_.empty()) return HIPCSTONE_OK;
+        HC_CHECK(hipEventRecord(pending_.back().stop, stream(s)));
+        return HIPCSTONE_OK;
+    }
+    hipcstone_status collect(double* ms, int32_t* cnt)
+    {
+        for (auto& r : pending_)
+        {
+            HC_CHECK(hipEventSynchronize(r.stop));
+            float t = 0;
+            HC_CHECK(hipEventElapsedTime(&t, r.start, r.stop));
+            ms_[r.stage] += t;
+            cnt_[r.stage] += 1;
+            pool_.push_back(r.start);
+            pool_.push_back(r.stop);
+        }
+        pending_.clear();
+        for (int i = 0; i < kStages; ++i)
+        {
+            if (ms) ms[i] = ms_[i];
+            if (cnt) cnt[i] = cnt_[i];
+        }
+        return HIPCSTONE_OK;
+    }
+
+private:
+    struct Rec
+    {
+        int stage;
+        hipEvent_t start;
+        hipEvent_t stop;
+    };
+    hipEvent_t get()
+    {
+        if (!pool_.empty())
+        {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    bool on_ = false;
+    double ms_[kStages]{};
+    int32_t cnt_[kStages]{};
+    std::vector<Rec> pending_;
+    std::vector<hipEvent_t> pool_;
+};
+
+struct PipelineBase
+{
+    StageTimer timer;
+    virtual ~PipelineBase() = default;
+    virtual hipcstone_status sync(hipcstone_stream s, const void* x, const void* y, const void* z, const void* h,
+                                  size_t n, const void* box, int fullBuild, uint32_t* nb, uint32_t* nc) = 0;
+    virtual void view(hipcstone_pipeline_view* v) const = 0;
+};
+
+// curve-dispatched wrappers around the per-type C ABI
+template<class K, class T>
+struct Api;
+
+#define HC_API(KS, K, RS, T)                                                                                           \
+    template<>                                                                                                         \
+    struct Api<K, T>                                                                                                   \
+    {                                                                                                                  \
+        using Box = hipcstone_box_##RS;                                                                                \
+        static hipcstone_status keys(int curve, hipcstone_stream s, const T* x, const T* y, const T* z, K* k,         \
+                                     size_t n, const Box* b)                                                           \
+        {                                                                                                              \
+            return curve == HIPCSTONE_HILBERT ? hipcstone_compute_sfc_keys_hilbert_##KS##_##RS(s, x, y, z, k, n, b, 0)\
+                                              : hipcstone_compute_sfc_keys_morton_##KS##_##RS(s, x, y, z, k, n, b, 0);\
+        }                                                                                                              \
+        static size_t sortWs(size_t n) { return hipcstone_sort_gather_workspace_size_##KS(n); }                        \
+        static hipcstone_status sortGather(hipcstone_stream s, K* k, uint32_t* p, size_t n, const T* xi, const T* yi, \
+                                           const T* zi, const T* hi, T* xo, T* yo, T* zo, T* ho, K* kb, uint32_t* vb, \
+                                           void* ws, size_t wsb)                                                       \
+        {                                                                                                              \
+            return hipcstone_sort_gather_##KS##_##RS(s, k, p, n, xi, yi, zi, hi, xo, yo, zo, ho, kb, vb, ws, wsb);    \
+        }                                                                                                              \
+        static size_t buildWs(int32_t L) { return hipcstone_build_octree_workspace_size_##KS(L); }                    \
+        static hipcstone_status update(hipcstone_stream s, const K* keys, size_t n, uint32_t bucket, const K* tree,   \
+                                       const uint32_t* counts, int32_t* nn, K* nt, uint32_t* nc, int32_t cap,          \
+                                       void* ws, size_t wsb, int* conv)                                                \
+        {                                                                                                              \
+            return hipcstone_update_octree_##KS(s, keys, n, bucket, tree, counts, nn, nt, nc, cap, 0xFFFFFFFFu, ws,   \
+                                                wsb, conv);                                                            \
+        }                                                                                                              \
+        static hipcstone_status build(hipcstone_stream s, const K* leaves, int32_t L, K* pf, int32_t* co,             \
+                                      int32_t* par, int32_t* lr, int32_t* i2l, int32_t* l2i, void* ws, size_t wsb,     \
+                                      int32_t* lrh)                                                                    \
+        {                                                                                                              \
+            return hipcstone_build_octree_##KS(s, leaves, L, pf, co, par, lr, i2l, l2i, ws, wsb, lrh);                \
+        }                                                                                                              \
+        static hipcstone_status centers(hipcstone_stream s, const K* pf, int32_t N, T* c, T* sz, const Box* b)        \
+        {                                                                                                              \
+            return hipcstone_compute_geo_centers_##KS##_##RS(s, pf, N, c, sz, b);                                     \
+        }                                                                                                              \
+        static hipcstone_status neighbors(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h,         \
+                                          uint32_t f, uint32_t l, const Box* b, const int32_t* co,                    \
+                                          const int32_t* par, const int32_t* i2l, const uint32_t* lay, const T* c,     \
+                                          const T* sz, int32_t N, uint32_t ngmax, uint32_t* nb, uint32_t* nc)          \
+        {                                                                                                              \
+            return hipcstone_find_neighbors_##RS(s, x, y, z, h, f, l, b, co, par, i2l, lay, c, sz, N, 1.0f, ngmax, nb,\
+                                                 nc);                                                                  \
+        }                                                                                                              \
+    };
+
+HC_API(u32, uint32_t, f32, float)
+HC_API(u32, uint32_t, f64, double)
+HC_API(u64, uint64_t, f32, float)
+HC_API(u64, uint64_t, f64, double)
+
+template<class K, class T>
+class Pipeline final : public PipelineBase
+{
+    using A   = Api<K, T>;
+    using Box = typename A::Box;
+
+public:
+    Pipeline(int curve, uint32_t bucket, uint32_t ngmax)
+        : curve_(curve)
+        , bucket_(bucket)
+        , ngmax_(ngmax)
+    {
+    }
+
+    hipcstone_status sync(hipcstone_stream s, const void* xv, const void* yv, const void* zv, const void* hv,
+                          size_t n, const void* boxv, int fullBuild, uint32_t* nb, uint32_t* nc) override
+    {
+        if (n == 0) return invalid("pipeline sync: no particles");
+        if (n > (size_t(1) << 30) - 1) return invalid("pipeline sync: more than 2^30-1 particles");
+        if (!xv || !yv || !zv || !hv || !boxv) return invalid("pipeline sync: null input");
+        const T* x = static_cast<const T*>(xv);
+        const T* y = static_cast<const T*>(yv);
+        const T* z = static_cast<const T*>(zv);
+        const T* h = static_cast<const T*>(hv);
+        box_       = *static_cast<const Box*>(boxv);
+
+        if (auto e = reserveParticles(n)) return e;
+        // gather into the particle set that does not alias the input
+        int out = 1 - cur_;
+        if (xyzh_[out][0].ptr == xv) out = cur_;
+        T* xo = xyzh_[out][0].template as<T>();
+        T* yo = xyzh_[out][1].template as<T>();
+        T* zo = xyzh_[out][2].template as<T>();
+        T* ho = xyzh_[out][3].template as<T>();
+
+        K* keys = keys_.as<K>();
+        StageTimer& tm = this->timer;
+#define HC_STAGE(id, call)                                                                                             \
+    do {                                                                                                               \
+        if (auto e_ = tm.begin(id, s)) return e_;                                                                     \
+        if (auto e_ = (call)) return e_;                                                                               \
+        if (auto e_ = tm.end(s)) return e_;                                                                            \
+    } while (0)
+        HC_STAGE(0, A::keys(curve_, s, x, y, z, keys, n, &box_));
+        HC_STAGE(1, A::sortGather(s, keys, perm_.as<uint32_t>(), n, x, y, z, h, xo, yo, zo, ho, keyBuf_.as<K>(),
+                                  valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes));
+        cur_ = out;
+        n_   = n;
+
+        if (fullBuild || numLeaves_ == 0) { HC_STAGE(2, computeOctree(s)); }
+        else
+        {
+            int conv = 0;
+            HC_STAGE(2, updateOnce(s, &conv));
+        }
+        HC_STAGE(3, link(s));
+        HC_STAGE(4, A::centers(s, prefixes_.as<K>(), numNodes_, centers_.as<T>(), sizes_.as<T>(), &box_));
+        HC_STAGE(5, hipcstone_leaf_layout(s, counts_[treeCur_].as<uint32_t>(), numLeaves_, layout_.as<uint32_t>(),
+                                          scanWs_.ptr, scanWs_.bytes));
+        if (nc)
+        {
+            HC_STAGE(6, A::neighbors(s, xo, yo, zo, ho, 0, uint32_t(n), &box_, childOffsets_.as<int32_t>(),
+                                     parents_.as<int32_t>(), internalToLeaf_.as<int32_t>(), layout_.as<uint32_t>(),
+                                     centers_.as<T>(), sizes_.as<T>(), numNodes_, nb ? ngmax_ : 0u, nb, nc));
+        }
+#undef HC_STAGE
+        return HIPCSTONE_OK;
+    }
+
+    void view(hipcstone_pipeline_view* v) const override
+    {
+        v->num_particles    = n_;
+        v->num_leaves       = numLeaves_;
+        v->num_nodes        = numNodes_;
+        v->num_internal     = numNodes_ - numLeaves_;
+        v->keys             = keys_.ptr;
+        v->perm             = perm_.ptr;
+        v->x                = xyzh_[cur_][0].ptr;
+        v->y                = xyzh_[cur_][1].ptr;
+        v->z                = xyzh_[cur_][2].ptr;
+        v->h                = xyzh_[cur_][3].ptr;
+        v->leaves           = tree_[treeCur_].ptr;
+        v->counts           = counts_[treeCur_].ptr;
+        v->prefixes         = prefixes_.ptr;
+        v->child_offsets    = childOffsets_.ptr;
+        v->parents          = parents_.ptr;
+        v->internal_to_leaf = internalToLeaf_.ptr;
+        v->leaf_to_internal = leafToInternal_.ptr;
+        v->level_range      = levelRange_.ptr;
+        v->centers          = centers_.ptr;
+        v->sizes            = sizes_.ptr;
+        v->layout           = layout_.ptr;
+        for (int i = 0; i < 24; ++i)
+            v->level_range_host[i] = i < int(maxLevel<K>() + 2) ? levelRangeHost_[i] : 0;
+        v->converged_calls = convergedCalls_;
+    }
+
+private:
+    hipcstone_status reserveParticles(size_t n)
+    {
+        if (n <= capN_) return HIPCSTONE_OK;
+        for (auto* b : {&keys_, &keyBuf_})
+            if (auto e = b->reserve(n * sizeof(K))) return e;
+        for (auto* b : {&perm_, &valBuf_})
+            if (auto e = b->reserve(n * sizeof(uint32_t))) return e;
+        for (int s = 0; s < 2; ++s)
+            for (int a = 0; a < 4; ++a)
+                if (auto e = xyzh_[s][a].reserve(n * sizeof(T))) return e;
+        if (auto e = sortWs_.reserve(A::sortWs(n))) return e;
+        capN_ = n;
+        return HIPCSTONE_OK;
+    }
+
+    // leaf-array buffers for `cap` leaves (tree ping-pong, counts, update workspace)
+    hipcstone_status reserveLeaves(int32_t cap)
+    {
+        if (cap <= capLeaves_) return HIPCSTONE_OK;
+        // keep the current tree across the reallocation
+        std::vector<K> keepTree;
+        std::vector<uint32_t> keepCounts;
+        if (numLeaves_ > 0)
+        {
+            keepTree.resize(numLeaves_ + 1);
+            keepCounts.resize(numLeaves_);
+            HC_CHECK(hipMemcpy(keepTree.data(), tree_[treeCur_].ptr, keepTree.size() * sizeof(K),
+                               hipMemcpyDeviceToHost));
+            HC_CHECK(hipMemcpy(keepCounts.data(), counts_[treeCur_].ptr, keepCounts.size() * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost));
+        }
+        int32_t c = std::max(cap, capLeaves_ * 2);
+        for (int t = 0; t < 2; ++t)
+        {
+            if (auto e = tree_[t].reserve((size_t(c) + 1) * sizeof(K))) return e;
+            if (auto e = counts_[t].reserve(size_t(c) * sizeof(uint32_t))) return e;
+        }
+        if (auto e = updWs_.reserve(hipcstone_update_octree_workspace_size(c))) return e;
+        if (numLeaves_ > 0)
+        {
+            HC_CHECK(hipMemcpy(tree_[treeCur_].ptr, keepTree.data(), keepTree.size() * sizeof(K),
+                               hipMemcpyHostToDevice));
+            HC_CHECK(hipMemcpy(counts_[treeCur_].ptr, keepCounts.data(), keepCounts.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice));
+        }
+        capLeaves_ = c;
+        return HIPCSTONE_OK;
+    }
+
+    // one updateOctree step from (tree_[treeCur_], counts_[treeCur_]); grows buffers and retries on CAPACITY
+    hipcstone_status updateOnce(hipcstone_stream s, int* converged)
+    {
+        for (int attempt = 0; attempt < 8; ++attempt)
+        {
+            int32_t nn  = numLeaves_;
+            int other   = 1 - treeCur_;
+            auto e      = A::update(s, keys_.as<K>(), n_, bucket_, tree_[treeCur_].template as<K>(),
+                                    counts_[treeCur_].template as<uint32_t>(), &nn, tree_[other].template as<K>(),
+                                    counts_[other].template as<uint32_t>(), capLeaves_, updWs_.ptr, updWs_.bytes,
+                                    converged);
+            if (e == HIPCSTONE_CAPACITY)
+            {
+                if (auto g = reserveLeaves(nn)) return g;
+                continue;
+            }
+            if (e) return e;
+            treeCur_   = other;
+            numLeaves_ = nn;
+            return HIPCSTONE_OK;
+        }
+        return invalid("pipeline: leaf capacity growth did not converge");
+    }
+
+    // computeOctree: root {0, nodeRange(0)} with counts {n}, update until the input tree was converged
+    hipcstone_status computeOctree(hipcstone_stream s)
+    {
+        if (auto e = reserveLeaves(std::max<int32_t>(capLeaves_, 4096))) return e;
+        K root[2]     = {K(0), nodeRange<K>(0)};
+        uint32_t cnt  = uint32_t(n_);
+        HC_CHECK(hipMemcpyAsync(tree_[treeCur_].ptr, root, sizeof(root), hipMemcpyHostToDevice, stream(s)));
+        HC_CHECK(hipMemcpyAsync(counts_[treeCur_].ptr, &cnt, sizeof(cnt), hipMemcpyHostToDevice, stream(s)));
+        numLeaves_  = 1;
+        int calls   = 0;
+        int conv    = 0;
+        while (!conv)
+        {
+            if (auto e = updateOnce(s, &conv)) return e;
+            if (++calls > 64) return invalid("pipeline: computeOctree did not converge in 64 steps");
+        }
+        convergedCalls_ = calls;
+        return HIPCSTONE_OK;
+    }
+
+    hipcstone_status link(hipcstone_stream s)
+    {
+        int32_t L = numLeaves_;
+        int32_t N = L + (L - 1) / 7;
+        if (N > capNodes_)
+        {
+            int32_t c = std::max(N, capNodes_ * 2);
+            size_t Nc = size_t(c);
+            if (auto e = prefixes_.reserve(Nc * sizeof(K))) return e;
+            for (auto* b : {&childOffsets_, &internalToLeaf_, &leafToInternal_})
+                if (auto e = b->reserve(Nc * sizeof(int32_t))) return e;
+            if (auto e = parents_.reserve(std::max<size_t>(1, (Nc - 1) / 8 + 1) * sizeof(int32_t))) return e;
+            if (auto e = levelRange_.reserve(32 * sizeof(int32_t))) return e;
+            if (auto e = centers_.reserve(Nc * 3 * sizeof(T))) return e;
+            if (auto e = sizes_.reserve(Nc * 3 * sizeof(T))) return e;
+            capNodes_ = c;
+        }
+        if (auto e = buildWs_.reserve(A::buildWs(L))) return e;
+        if (auto e = layout_.reserve((size_t(L) + 1) * sizeof(uint32_t))) return e;
+        if (auto e = scanWs_.reserve(hipcstone_scan_workspace_size(size_t(L)))) return e;
+        numNodes_ = N;
+        return A::build(s, tree_[treeCur_].template as<K>(), L, prefixes_.as<K>(), childOffsets_.as<int32_t>(),
+                        parents_.as<int32_t>(), levelRange_.as<int32_t>(), internalToLeaf_.as<int32_t>(),
+                        leafToInternal_.as<int32_t>(), buildWs_.ptr, buildWs_.bytes, levelRangeHost_);
+    }
+
+    int curve_;
+    uint32_t bucket_;
+    uint32_t ngmax_;
+    Box box_{};
+    size_t n_          = 0;
+    size_t capN_       = 0;
+    int32_t capLeaves_ = 0;
+    int32_t capNodes_  = 0;
+    int32_t numLeaves_ = 0;
+    int32_t numNodes_  = 0;
+    int cur_           = 0;
+    int treeCur_       = 0;
+    int convergedCalls_ = 0;
+    int32_t levelRangeHost_[32]{};
+
+    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_;
+    DeviceBuf xyzh_[2][4];
+    DeviceBuf tree_[2], counts_[2], updWs_;
+    DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
+    DeviceBuf centers_, sizes_, layout_, scanWs_;
+};
+
+} // namespace hipcstone
+
+struct hipcstone_pipeline
+{
+    std::unique_ptr<hipcstone::PipelineBase> impl;
+};
+
+using namespace hipcstone;
+
+extern "C"
+{
+
+hipcstone_status hipcstone_pipeline_create(hipcstone_pipeline** p, int curve, int key_bits, int real_bits,
+                                           uint32_t bucket_size, uint32_t ngmax)
+{
+    if (!p) return invalid("pipeline_create: null handle");
+    if (curve != HIPCSTONE_HILBERT && curve != HIPCSTONE_MORTON) return invalid("pipeline_create: curve");
+    if (bucket_size == 0) return invalid("pipeline_create: bucket_size must be positive");
+    auto* h = new hipcstone_pipeline;
+    if (key_bits == 64 && real_bits == 32) h->impl = std::make_unique<Pipeline<uint64_t, float>>(curve, bucket_size, ngmax);
+    else if (key_bits == 64 && real_bits == 64) h->impl = std::make_unique<Pipeline<uint64_t, double>>(curve, bucket_size, ngmax);
+    else if (key_bits == 32 && real_bits == 32) h->impl = std::make_unique<Pipeline<uint32_t, float>>(curve, bucket_size, ngmax);
+    else if (key_bits == 32 && real_bits == 64) h->impl = std::make_unique<Pipeline<uint32_t, double>>(curve, bucket_size, ngmax);
+    else
+    {
+        delete h;
+        return invalid("pipeline_create: key_bits must be 32|64 and real_bits 32|64");
+    }
+    *p = h;
+    return HIPCSTONE_OK;
+}
+
+hipcstone_status hipcstone_pipeline_destroy(hipcstone_pipeline* p)
+{
+    delete p;
+    return HIPCSTONE_OK;
+}
+
+hipcstone_status hipcstone_pipeline_sync(hipcstone_pipeline* p, hipcstone_stream s, const void* x, const void* y,
+                                         const void* z, const void* h, size_t n, const void* box, int full_build,
+                                         uint32_t* neighbors, uint32_t* neighbors_count)
+{
+    if (!p) return invalid("pipeline_sync: null handle");
+    return p->impl->sync(s, x, y, z, h, n, box, full_build, neighbors, neighbors_count);
+}
+
+hipcstone_status hipcstone_pipeline_set_timing(hipcstone_pipeline* p, int enable)
+{
+    if (!p) return invalid("pipeline_set_timing: null handle");
+    p->impl->timer.enable(enable != 0);
+    return HIPCSTONE_OK;
+}
+
+hipcstone_status hipcstone_pipeline_stage_times(hipcstone_pipeline* p, double* ms, int32_t* launches)
+{
+    if (!p) return invalid("pipeline_stage_times: null handle");
+    return p->impl->timer.collect(ms, launches);
+}
+
+hipcstone_status hipcstone_pipeline_get_view(const hipcstone_pipeline* p, hipcstone_pipeline_view* v)
+{
+    if (!p || !v) return invalid("pipeline_get_view: null pointer");
+    p->impl->view(v);
+    return HIPCSTONE_OK;
+}
+
+} // extern "C"

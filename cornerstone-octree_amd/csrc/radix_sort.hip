@@ -1,0 +1,522 @@
+// a4/a5: stable LSD radix sort (key + uint32 value) with a fused permutation gather.
+//
+// Replaces the reference's CUB DeviceRadixSort::SortPairs call (primitives/primitives_gpu.cu:322-356) and the
+// gather kernels (:74-90). Result contract: ascending keys, ties in input order (what std::stable_sort gives on
+// the CPU path, primitives/gather.hpp:42-67).
+//
+// Design (onesweep): one histogram kernel computes the digit histograms of all passes in a single read of the
+// keys; then each 8-bit digit pass is a single kernel that
+//   1. takes a tile id from an atomic counter (ids follow launch order, so every tile waits only on tiles that
+//      have already started: the look-back always terminates),
+//   2. ranks its 4096 keys stably with wave64 match-by-ballot (8 ballots per key) and per-wave LDS digit counters,
+//   3. publishes its per-digit counts and resolves its global digit offsets by decoupled look-back
+//      (one thread per digit, status word = 2-bit flag | 30-bit count),
+//   4. reorders the tile in LDS and writes each digit run contiguously.
+// The last pass optionally writes the permutation and gathers x,y,z,h through it.
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace hipcstone
+{
+
+namespace sortcfg
+{
+constexpr int kThreads   = 256;
+constexpr int kItems     = 16;
+constexpr int kTile      = kThreads * kItems; // 4096 keys per tile
+constexpr int kRadixBits = 8;
+constexpr int kRadix     = 1 << kRadixBits;
+constexpr int kWaves     = kThreads / kWave;
+constexpr int kMaxPasses = 8;
+constexpr int kHistBlocks = 1024;
+
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix    = 2u << 30;
+constexpr uint32_t kValueMask     = (1u << 30) - 1;
+} // namespace sortcfg
+
+using namespace sortcfg;
+
+// Workspace layout (bytes, 256-aligned pieces):
+//   hist    [kMaxPasses][kRadix] u32   digit histograms, then exclusive digit offsets
+//   counter [kMaxPasses] u32 (padded) tile-id counters
+//   status  [kMaxPasses][numTiles][kRadix] u32 look-back words
+struct SortWorkspace
+{
+    uint32_t* hist;
+    uint32_t* counter;
+    uint32_t* status;
+    size_t bytes;
+};
+
+static size_t alignUp(size_t v) { return (v + 255) & ~size_t(255); }
+
+static size_t numTiles(size_t n) { return (n + kTile - 1) / kTile; }
+
+static SortWorkspace carveWorkspace(void* base, size_t n, int passes)
+{
+    SortWorkspace w{};
+    size_t off = 0;
+    char* b    = static_cast<char*>(base);
+    w.hist     = reinterpret_cast<uint32_t*>(b + off);
+    off += alignUp(sizeof(uint32_t) * kMaxPasses * kRadix);
+    w.counter = reinterpret_cast<uint32_t*>(b + off);
+    off += alignUp(sizeof(uint32_t) * kMaxPasses);
+    w.status = reinterpret_cast<uint32_t*>(b + off);
+    off += alignUp(sizeof(uint32_t) * size_t(passes) * numTiles(n) * kRadix);
+    w.bytes = off;
+    return w;
+}
+
+static size_t workspaceBytes(size_t n, int passes)
+{
+    return carveWorkspace(nullptr, n, passes).bytes;
+}
+
+__device__ inline uint32_t atomicLoadRelaxed(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void atomicStoreRelaxed(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Digit histograms of all passes [0, numPasses) in one read of the keys. hist must be zeroed.
+template<class K>
+__global__ __launch_bounds__(kThreads) void histogramKernel(const K* __restrict__ keys, size_t n, int numPasses,
+                                                            uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t sh[kMaxPasses * kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kThreads)
+        sh[i] = 0;
+    __syncthreads();
+
+    const int lane = laneId();
+    size_t stride  = size_t(gridDim.x) * kThreads;
+    for (size_t base = size_t(blockIdx.x) * kThreads; base < n; base += stride)
+    {
+        size_t i     = base + threadIdx.x;
+        bool valid   = i < n;
+        K key        = valid ? keys[i] : K(0);
+        uint64_t act = ballot(valid);
+        if (act == 0) continue;
+        int first = __builtin_ctzll(act);
+        for (int p = 0; p < numPasses; ++p)
+        {
+            unsigned d      = unsigned(key >> (p * kRadixBits)) & (kRadix - 1);
+            unsigned d0     = unsigned(__builtin_amdgcn_readlane(int(d), first));
+            uint64_t same   = ballot(valid && d == d0);
+            if (same == act)
+            {
+                // whole wave shares the digit (typical for high digits of nearly sorted keys)
+                if (lane == first) atomicAdd(&sh[p * kRadix + d], uint32_t(__popcll(act)));
+            }
+            else if (valid) { atomicAdd(&sh[p * kRadix + d], 1u); }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < numPasses * kRadix; i += kThreads)
+    {
+        if (sh[i]) atomicAdd(&hist[i], sh[i]);
+    }
+}
+
+// Exclusive scan of each pass's 256-bin histogram, in place. One block of kRadix threads.
+__global__ __launch_bounds__(kRadix) void scanHistogramKernel(uint32_t* hist, int numPasses)
+{
+    __shared__ uint32_t sh[kRadix];
+    for (int p = 0; p < numPasses; ++p)
+    {
+        uint32_t v = hist[p * kRadix + threadIdx.x];
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        // Hillis-Steele inclusive scan over 256 values
+        for (int off = 1; off < kRadix; off <<= 1)
+        {
+            uint32_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        hist[p * kRadix + threadIdx.x] = sh[threadIdx.x] - v;
+        __syncthreads();
+    }
+}
+
+template<class T>
+struct GatherArgs
+{
+    const T* xIn;
+    const T* yIn;
+    const T* zIn;
+    const T* hIn;
+    T* xOut;
+    T* yOut;
+    T* zOut;
+    T* hOut;
+};
+
+// ValueMode: 0 = keys only, 1 = values read from valuesIn, 2 = values are the input positions (iota)
+// Gather: if GT is not void-like (GatherOn), the pass also writes xOut[dst] = xIn[value] etc.
+template<class K, int ValueMode, bool GatherOn, class GT>
+__global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restrict__ keysIn, K* __restrict__ keysOut,
+                                                               const uint32_t* __restrict__ valuesIn,
+                                                               uint32_t* __restrict__ valuesOut, size_t n, int shift,
+                                                               int digitBits, const uint32_t* __restrict__ digitOffsets,
+                                                               uint32_t* __restrict__ tileCounter,
+                                                               uint32_t* __restrict__ status, GatherArgs<GT> g)
+{
+    __shared__ K shKeys[kTile];
+    __shared__ uint32_t shVals[ValueMode != 0 ? kTile : 1];
+    __shared__ uint32_t shWaveHist[kWaves][kRadix];
+    __shared__ uint32_t shLocalStart[kRadix];
+    __shared__ uint32_t shGlobalBase[kRadix];
+    __shared__ uint32_t shTileId;
+    __shared__ uint32_t shWaveSums[kWaves];
+
+    const int tid   = threadIdx.x;
+    const int lane  = laneId();
+    const int wave  = tid / kWave;
+    const uint32_t digitMask = (1u << digitBits) - 1u;
+
+    if (tid == 0) shTileId = atomicAdd(tileCounter, 1u);
+    for (int i = tid; i < kWaves * kRadix; i += kThreads)
+        (&shWaveHist[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = shTileId;
+    const size_t tileBase = size_t(tile) * kTile;
+
+    // ---- load (striped per wave: item k of lane l at waveBase + k*64 + l) ----
+    K keys[kItems];
+    uint32_t vals[kItems];
+    uint32_t ranks[kItems];
+    const size_t waveBase = tileBase + size_t(wave) * kWave * kItems;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        size_t i = waveBase + size_t(k) * kWave + lane;
+        bool v   = i < n;
+        keys[k]  = v ? keysIn[i] : K(0);
+        if constexpr (ValueMode == 1) { vals[k] = v ? valuesIn[i] : 0u; }
+        else if constexpr (ValueMode == 2) { vals[k] = uint32_t(i); }
+    }
+
+    // ---- stable rank within the wave: match lanes with equal digits by ballots ----
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        size_t i       = waveBase + size_t(k) * kWave + lane;
+        bool v         = i < n;
+        unsigned d     = unsigned(keys[k] >> shift) & digitMask;
+        uint64_t peers = ballot(v);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b)
+        {
+            if (b < digitBits)
+            {
+                bool bit     = (d >> b) & 1u;
+                uint64_t bal = ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+        }
+        unsigned before = laneRank(peers);
+        uint32_t prev   = v ? shWaveHist[wave][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (v && before == 0) shWaveHist[wave][d] = prev + uint32_t(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+        ranks[k] = prev + before;
+    }
+    __syncthreads();
+
+    // ---- per digit: counts of the tile, exclusive offsets of each wave ----
+    uint32_t tileCount = 0;
+    {
+        const int d = tid; // kThreads == kRadix
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+        {
+            uint32_t c       = shWaveHist[w][d];
+            shWaveHist[w][d] = run;
+            run += c;
+        }
+        tileCount = run;
+    }
+
+    // publish this tile's counts for the look-back before scanning locally
+    uint32_t* myStatus = status + size_t(tile) * kRadix;
+    if (tile == 0) atomicStoreRelaxed(myStatus + tid, kFlagPrefix | tileCount);
+    else atomicStoreRelaxed(myStatus + tid, kFlagAggregate | tileCount);
+
+    // ---- block exclusive scan of tileCount over the 256 digits ----
+    uint32_t incl = tileCount;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1)
+    {
+        uint32_t t = uint32_t(__shfl_up(int(incl), off, kWave));
+        if (lane >= off) incl += t;
+    }
+    if (lane == kWave - 1) shWaveSums[wave] = incl;
+
+    // ---- decoupled look-back: exclusive prefix of this digit over all earlier tiles ----
+    uint32_t exclusive = 0;
+    if (tile > 0)
+    {
+        int64_t j = int64_t(tile) - 1;
+        uint32_t spins = 0;
+        while (j >= 0)
+        {
+            uint32_t s = atomicLoadRelaxed(status + size_t(j) * kRadix + tid);
+            uint32_t flag = s & ~kValueMask;
+            if (flag == 0)
+            {
+                // predecessor has not published yet; every tile publishes before it waits, so this ends. The cap
+                // only guards against a corrupted workspace.
+                if (++spins > (1u << 22)) break;
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            exclusive += s & kValueMask;
+            if (flag == kFlagPrefix) break;
+            --j;
+        }
+        atomicStoreRelaxed(myStatus + tid, kFlagPrefix | (exclusive + tileCount));
+    }
+    __syncthreads();
+    uint32_t waveOffset = 0;
+    for (int w = 0; w < wave; ++w)
+        waveOffset += shWaveSums[w];
+    uint32_t localStart   = waveOffset + incl - tileCount;
+    shLocalStart[tid]     = localStart;
+    shGlobalBase[tid]     = digitOffsets[tid] + exclusive - localStart;
+    __syncthreads();
+
+    // ---- scatter into LDS in sorted order ----
+    uint32_t validInTile = n > tileBase ? uint32_t(min(size_t(kTile), n - tileBase)) : 0u;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        size_t i = waveBase + size_t(k) * kWave + lane;
+        if (i < n)
+        {
+            unsigned d   = unsigned(keys[k] >> shift) & digitMask;
+            uint32_t pos = shLocalStart[d] + shWaveHist[wave][d] + ranks[k];
+            shKeys[pos]  = keys[k];
+            if constexpr (ValueMode != 0) shVals[pos] = vals[k];
+        }
+    }
+    __syncthreads();
+
+    // ---- write out: consecutive LDS slots of one digit go to consecutive global slots ----
+    for (uint32_t i = tid; i < validInTile; i += kThreads)
+    {
+        K key        = shKeys[i];
+        unsigned d   = unsigned(key >> shift) & digitMask;
+        uint32_t dst = shGlobalBase[d] + i; // modulo 2^32: the base alone may wrap below zero
+        keysOut[dst] = key;
+        if constexpr (ValueMode != 0)
+        {
+            uint32_t v     = shVals[i];
+            valuesOut[dst] = v;
+            if constexpr (GatherOn)
+            {
+                g.xOut[dst] = g.xIn[v];
+                g.yOut[dst] = g.yIn[v];
+                g.zOut[dst] = g.zIn[v];
+                if (g.hIn) g.hOut[dst] = g.hIn[v];
+            }
+        }
+    }
+}
+
+template<class T>
+__global__ void gatherKernel(const uint32_t* __restrict__ perm, size_t n, const T* __restrict__ src,
+                             T* __restrict__ dst)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ void fillU32Kernel(uint32_t* p, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+static hipcstone_status zeroU32(uint32_t* p, size_t n, hipStream_t s)
+{
+    if (n == 0) return HIPCSTONE_OK;
+    fillU32Kernel<<<ceilDiv(n, 256), 256, 0, s>>>(p, n);
+    HC_CHECK_LAUNCH("fillU32Kernel");
+    return HIPCSTONE_OK;
+}
+
+template<class K>
+static int numPassesFor(int endBit)
+{
+    return (endBit + kRadixBits - 1) / kRadixBits;
+}
+
+// Generic driver. valueMode 0/1/2 as in onesweepPassKernel. When gather is set, the last pass gathers.
+template<class K, class GT>
+static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size_t n, K* keyBuf, uint32_t* valueBuf,
+                                  void* workspace, size_t wsBytes, int endBit, int valueMode, const GatherArgs<GT>* g)
+{
+    if (n == 0) return HIPCSTONE_OK;
+    if (n > kValueMask) return invalid("radix sort: more than 2^30-1 keys");
+    if (endBit <= 0 || endBit > int(8 * sizeof(K))) endBit = int(8 * sizeof(K));
+    int passes = numPassesFor<K>(endBit);
+    size_t need = workspaceBytes(n, passes);
+    if (!workspace || wsBytes < need) return invalid("radix sort: workspace too small");
+    if (!keys || !keyBuf) return invalid("radix sort: null key buffer");
+    if (valueMode != 0 && (!values || !valueBuf)) return invalid("radix sort: null value buffer");
+
+    SortWorkspace w = carveWorkspace(workspace, n, passes);
+    size_t tiles    = numTiles(n);
+    if (auto st = zeroU32(w.hist, size_t(kMaxPasses) * kRadix, s)) return st;
+    if (auto st = zeroU32(w.counter, kMaxPasses, s)) return st;
+    if (auto st = zeroU32(w.status, size_t(passes) * tiles * kRadix, s)) return st;
+
+    unsigned hb = unsigned(std::min<size_t>(kHistBlocks, (n + kThreads - 1) / kThreads));
+    histogramKernel<K><<<hb, kThreads, 0, s>>>(keys, n, passes, w.hist);
+    HC_CHECK_LAUNCH("histogramKernel");
+    scanHistogramKernel<<<1, kRadix, 0, s>>>(w.hist, passes);
+    HC_CHECK_LAUNCH("scanHistogramKernel");
+
+    // ping-pong so that the last pass writes into keys/values
+    K* src         = keys;
+    uint32_t* vsrc = values;
+    K* dst;
+    uint32_t* vdst;
+    GatherArgs<GT> noGather{};
+    for (int p = 0; p < passes; ++p)
+    {
+        bool last = p == passes - 1;
+        // destination alternates; arrange the final destination to be keys
+        bool toBuf = ((passes - 1 - p) % 2) == 1;
+        dst        = toBuf ? keyBuf : keys;
+        vdst       = toBuf ? valueBuf : values;
+        int bits   = std::min(kRadixBits, endBit - p * kRadixBits);
+        int mode   = (p == 0 && valueMode == 2) ? 2 : (valueMode == 0 ? 0 : 1);
+        const uint32_t* off = w.hist + p * kRadix;
+        uint32_t* cnt       = w.counter + p;
+        uint32_t* st        = w.status + size_t(p) * tiles * kRadix;
+        // an odd number of passes would make pass 0 read and write keys; stage through the buffer first
+        if (src == dst)
+        {
+            HC_CHECK(hipMemcpyAsync(keyBuf, keys, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+            src = keyBuf;
+            if (mode == 1)
+            {
+                HC_CHECK(hipMemcpyAsync(valueBuf, values, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+                vsrc = valueBuf;
+            }
+        }
+        bool gatherNow = last && g != nullptr;
+        if (mode == 0)
+            onesweepPassKernel<K, 0, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
+                src, dst, nullptr, nullptr, n, p * kRadixBits, bits, off, cnt, st, noGather);
+        else if (mode == 2 && !gatherNow)
+            onesweepPassKernel<K, 2, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
+                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather);
+        else if (mode == 2 && gatherNow)
+            onesweepPassKernel<K, 2, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
+                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, *g);
+        else if (!gatherNow)
+            onesweepPassKernel<K, 1, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
+                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather);
+        else
+            onesweepPassKernel<K, 1, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
+                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, *g);
+        HC_CHECK_LAUNCH("onesweepPassKernel");
+        src  = dst;
+        vsrc = vdst;
+    }
+    return HIPCSTONE_OK;
+}
+
+template<class K>
+static size_t sortWorkspaceSize(size_t n)
+{
+    return workspaceBytes(n, numPassesFor<K>(int(8 * sizeof(K))));
+}
+
+template<class T>
+static hipcstone_status gather(hipcstone_stream s, const uint32_t* perm, size_t n, const T* src, T* dst)
+{
+    if (n == 0) return HIPCSTONE_OK;
+    if (!perm || !src || !dst) return invalid("gather: null pointer");
+    gatherKernel<T><<<ceilDiv(n, 256), 256, 0, stream(s)>>>(perm, n, src, dst);
+    HC_CHECK_LAUNCH("gatherKernel");
+    return HIPCSTONE_OK;
+}
+
+template<class K, class T>
+static hipcstone_status sortGather(hipcstone_stream s, K* keys, uint32_t* perm, size_t n, const T* xIn, const T* yIn,
+                                   const T* zIn, const T* hIn, T* xOut, T* yOut, T* zOut, T* hOut, K* keyBuf,
+                                   uint32_t* valueBuf, void* ws, size_t wsBytes)
+{
+    if (n == 0) return HIPCSTONE_OK;
+    if (!xIn || !yIn || !zIn || !xOut || !yOut || !zOut || (hIn && !hOut)) return invalid("sortGather: null array");
+    GatherArgs<T> g{xIn, yIn, zIn, hIn, xOut, yOut, zOut, hOut};
+    return radixSort<K, T>(stream(s), keys, perm, n, keyBuf, valueBuf, ws, wsBytes, int(8 * sizeof(K)), 2, &g);
+}
+
+} // namespace hipcstone
+
+using namespace hipcstone;
+
+extern "C"
+{
+
+size_t hipcstone_sort_by_key_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n); }
+size_t hipcstone_sort_by_key_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n); }
+size_t hipcstone_sort_gather_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n); }
+size_t hipcstone_sort_gather_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n); }
+
+hipcstone_status hipcstone_sort_by_key_u32(hipcstone_stream s, uint32_t* keys, uint32_t* values, size_t n,
+                                           uint32_t* key_buf, uint32_t* value_buf, void* workspace,
+                                           size_t workspace_bytes, int end_bit)
+{
+    return radixSort<uint32_t, float>(stream(s), keys, values, n, key_buf, value_buf, workspace, workspace_bytes,
+                                      end_bit, values ? 1 : 0, nullptr);
+}
+
+hipcstone_status hipcstone_sort_by_key_u64(hipcstone_stream s, uint64_t* keys, uint32_t* values, size_t n,
+                                           uint64_t* key_buf, uint32_t* value_buf, void* workspace,
+                                           size_t workspace_bytes, int end_bit)
+{
+    return radixSort<uint64_t, float>(stream(s), keys, values, n, key_buf, value_buf, workspace, workspace_bytes,
+                                      end_bit, values ? 1 : 0, nullptr);
+}
+
+#define HIPCSTONE_DEF_SORTGATHER(KS, K, RS, T)                                                                         \
+    hipcstone_status hipcstone_sort_gather_##KS##_##RS(                                                                \
+        hipcstone_stream s, K* keys, uint32_t* perm, size_t n, const T* x_in, const T* y_in, const T* z_in,           \
+        const T* h_in, T* x_out, T* y_out, T* z_out, T* h_out, K* key_buf, uint32_t* value_buf, void* workspace,       \
+        size_t workspace_bytes)                                                                                        \
+    {                                                                                                                  \
+        return sortGather<K, T>(s, keys, perm, n, x_in, y_in, z_in, h_in, x_out, y_out, z_out, h_out, key_buf,        \
+                                value_buf, workspace, workspace_bytes);                                                \
+    }
+
+HIPCSTONE_DEF_SORTGATHER(u32, uint32_t, f32, float)
+HIPCSTONE_DEF_SORTGATHER(u32, uint32_t, f64, double)
+HIPCSTONE_DEF_SORTGATHER(u64, uint64_t, f32, float)
+HIPCSTONE_DEF_SORTGATHER(u64, uint64_t, f64, double)
+
+hipcstone_status hipcstone_gather_u32(hipcstone_stream s, const uint32_t* perm, size_t n, const uint32_t* src,
+                                      uint32_t* dst)
+{
+    return gather<uint32_t>(s, perm, n, src, dst);
+}
+
+hipcstone_status hipcstone_gather_u64(hipcstone_stream s, const uint32_t* perm, size_t n, const uint64_t* src,
+                                      uint64_t* dst)
+{
+    return gather<uint64_t>(s, perm, n, src, dst);
+}
+
+} // extern "C"

@@ -1,0 +1,45 @@
+// Host-side helpers shared by the C ABI translation units: error capture and launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "hipcstone.h"
+
+namespace hipcstone
+{
+
+// Record a message for hipcstone_last_error() on this host thread.
+void setError(const std::string& msg);
+
+inline hipcstone_status hipFail(hipError_t e, const char* what)
+{
+    setError(std::string(what) + ": " + hipGetErrorString(e));
+    return HIPCSTONE_HIP_ERROR;
+}
+
+inline hipcstone_status invalid(const char* what)
+{
+    setError(std::string("invalid argument: ") + what);
+    return HIPCSTONE_INVALID_ARG;
+}
+
+inline hipStream_t stream(hipcstone_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+} // namespace hipcstone
+
+// Return the status of a failed HIP call from the enclosing C ABI function.
+#define HC_CHECK(call)                                                                                                 \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return ::hipcstone::hipFail(e_, #call);                                                  \
+    } while (0)
+
+// Check that the last kernel launch was accepted.
+#define HC_CHECK_LAUNCH(name)                                                                                          \
+    do {                                                                                                               \
+        hipError_t e_ = hipGetLastError();                                                                             \
+        if (e_ != hipSuccess) return ::hipcstone::hipFail(e_, name);                                                   \
+    } while (0)

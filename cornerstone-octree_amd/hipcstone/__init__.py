@@ -1,0 +1,489 @@
+"""Python binding of libhipcstone (the MI355X-native single-domain octree pipeline) over its C ABI.
+
+The product is the C ABI in include/hipcstone.h, implemented by lib/libhipcstone.so (HIP kernels for gfx950 plus
+C++ host orchestration). This module loads that library with ctypes and exposes:
+
+* ``DeviceArray`` — device memory owned from Python (hipMalloc through the library), with numpy upload/download;
+* ``make_box`` — the reference's Box<T> value (sfc/box.hpp:95-175) as the C struct;
+* one function per C ABI entry (keys, sort, counts, rebalance, update, link, centers, layout, bbox, halos,
+  neighbors) taking DeviceArrays;
+* ``Pipeline`` — the plain single-domain sync step (keys -> sort -> tree -> link -> centers -> layout ->
+  neighbors).
+
+There is no CPU fallback: if the library is missing, importing this module raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhipcstone.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libhipcstone.so not found at {LIB_PATH}; build it with `make -C cornerstone-octree_amd` "
+        "(or __graft_entry__.build())"
+    )
+
+_lib = C.CDLL(LIB_PATH)
+
+HILBERT = 1
+MORTON = 0
+OPEN, PERIODIC, FIXED, CUBIC_OPEN = 0, 1, 2, 3
+
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP_ERROR", 4: "CAPACITY"}
+
+_lib.hipcstone_last_error.restype = C.c_char_p
+_lib.hipcstone_build_info.restype = C.c_char_p
+
+
+class HipcstoneError(RuntimeError):
+    def __init__(self, status, where):
+        msg = _lib.hipcstone_last_error().decode(errors="replace")
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _check(status, where):
+    if status != 0:
+        raise HipcstoneError(status, where)
+
+
+def build_info() -> str:
+    return _lib.hipcstone_build_info().decode()
+
+
+# ---------------------------------------------------------------- boxes
+
+
+class BoxF32(C.Structure):
+    _fields_ = [("lim", C.c_float * 6), ("len", C.c_float * 3), ("ilen", C.c_float * 3), ("bnd", C.c_uint8 * 3)]
+
+
+class BoxF64(C.Structure):
+    _fields_ = [("lim", C.c_double * 6), ("len", C.c_double * 3), ("ilen", C.c_double * 3), ("bnd", C.c_uint8 * 3)]
+
+
+def make_box(lo, hi, dtype=np.float32, boundary=(OPEN, OPEN, OPEN)):
+    """Box(xmin,xmax,ymin,ymax,zmin,zmax,bx,by,bz) as built by the reference constructor (box.hpp:100-122).
+
+    ``lo``/``hi`` are scalars (cube) or 3-sequences.
+    """
+    lo = np.broadcast_to(np.asarray(lo, dtype=np.float64), (3,))
+    hi = np.broadcast_to(np.asarray(hi, dtype=np.float64), (3,))
+    b = [int(v) for v in boundary]
+    if np.dtype(dtype) == np.float32:
+        box = BoxF32()
+        _lib.hipcstone_make_box_f32(*(C.c_float(float(np.float32(v))) for v in (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])),
+                                    *b, C.byref(box))
+    else:
+        box = BoxF64()
+        _lib.hipcstone_make_box_f64(*(C.c_double(float(v)) for v in (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])),
+                                    *b, C.byref(box))
+    return box
+
+
+def box_dtype(box):
+    return np.float32 if isinstance(box, BoxF32) else np.float64
+
+
+# ---------------------------------------------------------------- device memory and streams
+
+
+class DeviceArray:
+    """A device allocation (hipMalloc via the library) with a numpy dtype and element count."""
+
+    def __init__(self, count, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        self._ptr = C.c_void_p()
+        _check(_lib.hipcstone_device_alloc(C.byref(self._ptr), C.c_size_t(max(self.nbytes, 1))), "device_alloc")
+
+    @property
+    def ptr(self):
+        return self._ptr.value
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size <= self.count
+        _check(_lib.hipcstone_memcpy_h2d(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes),
+                                         _stream(stream)), "memcpy_h2d")
+
+    def download(self, count=None, dtype=None, stream=None):
+        dt = np.dtype(dtype) if dtype is not None else self.dtype
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        _check(_lib.hipcstone_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), C.c_size_t(out.nbytes),
+                                         _stream(stream)), "memcpy_d2h")
+        return out
+
+    def fill(self, byte=0, stream=None):
+        _check(_lib.hipcstone_fill_bytes(C.c_void_p(self.ptr), C.c_uint8(byte), C.c_size_t(self.nbytes),
+                                         _stream(stream)), "fill_bytes")
+
+    def free(self):
+        if self._ptr and self._ptr.value:
+            _lib.hipcstone_device_free(self._ptr)
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download_ptr(ptr, count, dtype, stream=None):
+    dt = np.dtype(dtype)
+    out = np.empty(int(count), dtype=dt)
+    if out.nbytes:
+        _check(_lib.hipcstone_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(out.nbytes),
+                                         _stream(stream)), "memcpy_d2h")
+    return out
+
+
+def _stream(s):
+    if s is None:
+        return C.c_void_p(0)
+    return C.c_void_p(s.handle if isinstance(s, Stream) else s)
+
+
+def _p(a):
+    if a is None:
+        return C.c_void_p(0)
+    if isinstance(a, DeviceArray):
+        return C.c_void_p(a.ptr)
+    return C.c_void_p(int(a))
+
+
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(_lib.hipcstone_stream_create(C.byref(h)), "stream_create")
+        self.handle = h.value
+
+    def sync(self):
+        _check(_lib.hipcstone_stream_sync(C.c_void_p(self.handle)), "stream_sync")
+
+    def __del__(self):
+        try:
+            _lib.hipcstone_stream_destroy(C.c_void_p(self.handle))
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(_lib.hipcstone_event_create(C.byref(h)), "event_create")
+        self.handle = h.value
+
+    def record(self, stream=None):
+        _check(_lib.hipcstone_event_record(C.c_void_p(self.handle), _stream(stream)), "event_record")
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        _check(_lib.hipcstone_event_elapsed_ms(C.c_void_p(self.handle), C.c_void_p(later.handle), C.byref(ms)),
+               "event_elapsed")
+        return ms.value
+
+    def __del__(self):
+        try:
+            _lib.hipcstone_event_destroy(C.c_void_p(self.handle))
+        except Exception:
+            pass
+
+
+def device_sync():
+    _check(_lib.hipcstone_device_sync(), "device_sync")
+
+
+# ---------------------------------------------------------------- kernels
+
+_KS = {np.dtype(np.uint32): "u32", np.dtype(np.uint64): "u64"}
+_RS = {np.dtype(np.float32): "f32", np.dtype(np.float64): "f64"}
+
+
+def _fn(name):
+    return getattr(_lib, name)
+
+
+def compute_sfc_keys(x, y, z, keys, box, curve=HILBERT, honor_remove_key=False, stream=None):
+    """keys[i] = sfc3D(x[i], y[i], z[i], box) (sfc/sfc.hpp:142-179, sfc_gpu.cu:24-33)."""
+    cname = "hilbert" if curve == HILBERT else "morton"
+    f = _fn(f"hipcstone_compute_sfc_keys_{cname}_{_KS[keys.dtype]}_{_RS[x.dtype]}")
+    _check(f(_stream(stream), _p(x), _p(y), _p(z), _p(keys), C.c_size_t(x.count), C.byref(box),
+             C.c_int(int(honor_remove_key))), "compute_sfc_keys")
+
+
+def sort_by_key_workspace_size(n, key_dtype):
+    f = _fn(f"hipcstone_sort_by_key_workspace_size_{_KS[np.dtype(key_dtype)]}")
+    f.restype = C.c_size_t
+    return f(C.c_size_t(n))
+
+
+def sort_by_key(keys, values, n=None, end_bit=0, stream=None):
+    """Stable ascending sort of keys (in place) carrying values (primitives_gpu.h:119-130)."""
+    n = keys.count if n is None else n
+    ws = DeviceArray(sort_by_key_workspace_size(n, keys.dtype), np.uint8)
+    kb = DeviceArray(n, keys.dtype)
+    vb = DeviceArray(n, np.uint32) if values is not None else None
+    f = _fn(f"hipcstone_sort_by_key_{_KS[keys.dtype]}")
+    _check(f(_stream(stream), _p(keys), _p(values), C.c_size_t(n), _p(kb), _p(vb), _p(ws), C.c_size_t(ws.nbytes),
+             C.c_int(end_bit)), "sort_by_key")
+    device_sync()
+
+
+def sort_gather(keys, perm, x, y, z, h, xo, yo, zo, ho, stream=None):
+    n = keys.count
+    f = _fn(f"hipcstone_sort_gather_workspace_size_{_KS[keys.dtype]}")
+    f.restype = C.c_size_t
+    ws = DeviceArray(f(C.c_size_t(n)), np.uint8)
+    kb = DeviceArray(n, keys.dtype)
+    vb = DeviceArray(n, np.uint32)
+    g = _fn(f"hipcstone_sort_gather_{_KS[keys.dtype]}_{_RS[x.dtype]}")
+    _check(g(_stream(stream), _p(keys), _p(perm), C.c_size_t(n), _p(x), _p(y), _p(z), _p(h), _p(xo), _p(yo), _p(zo),
+             _p(ho), _p(kb), _p(vb), _p(ws), C.c_size_t(ws.nbytes)), "sort_gather")
+    device_sync()
+
+
+def gather(perm, src, dst, stream=None):
+    f = _fn("hipcstone_gather_u32" if src.dtype.itemsize == 4 else "hipcstone_gather_u64")
+    _check(f(_stream(stream), _p(perm), C.c_size_t(perm.count), _p(src), _p(dst)), "gather")
+
+
+def compute_node_counts(tree, counts, num_nodes, keys, num_keys=None, max_count=0xFFFFFFFF, stream=None):
+    num_keys = keys.count if num_keys is None else num_keys
+    f = _fn(f"hipcstone_compute_node_counts_{_KS[tree.dtype]}")
+    _check(f(_stream(stream), _p(tree), _p(counts), C.c_int32(num_nodes), _p(keys), C.c_size_t(num_keys),
+             C.c_uint32(max_count), C.c_int(1)), "compute_node_counts")
+
+
+def compute_node_ops(tree, num_nodes, counts, bucket_size, node_ops, stream=None):
+    """Returns (new_num_nodes, converged); node_ops receives the exclusive scan of the ops (n+1 entries)."""
+    _lib.hipcstone_node_ops_workspace_size.restype = C.c_size_t
+    ws = DeviceArray(_lib.hipcstone_node_ops_workspace_size(C.c_int32(num_nodes)), np.uint8)
+    nn = C.c_int32()
+    conv = C.c_int()
+    f = _fn(f"hipcstone_compute_node_ops_{_KS[tree.dtype]}")
+    _check(f(_stream(stream), _p(tree), C.c_int32(num_nodes), _p(counts), C.c_uint32(bucket_size), _p(node_ops),
+             _p(ws), C.c_size_t(ws.nbytes), C.byref(nn), C.byref(conv)), "compute_node_ops")
+    return nn.value, bool(conv.value)
+
+
+def rebalance_tree(tree, num_nodes, new_num_nodes, node_ops, new_tree, stream=None):
+    f = _fn(f"hipcstone_rebalance_tree_{_KS[tree.dtype]}")
+    _check(f(_stream(stream), _p(tree), C.c_int32(num_nodes), C.c_int32(new_num_nodes), _p(node_ops), _p(new_tree)),
+           "rebalance_tree")
+
+
+def update_octree(keys, bucket_size, tree, counts, num_nodes, new_tree, new_counts, capacity, max_count=0xFFFFFFFF,
+                  stream=None):
+    """One updateOctree step (update_gpu.cuh:43-68). Returns (new_num_nodes, converged)."""
+    _lib.hipcstone_update_octree_workspace_size.restype = C.c_size_t
+    ws = DeviceArray(_lib.hipcstone_update_octree_workspace_size(C.c_int32(num_nodes)), np.uint8)
+    nn = C.c_int32(num_nodes)
+    conv = C.c_int()
+    f = _fn(f"hipcstone_update_octree_{_KS[keys.dtype]}")
+    st = f(_stream(stream), _p(keys), C.c_size_t(keys.count), C.c_uint32(bucket_size), _p(tree), _p(counts),
+           C.byref(nn), _p(new_tree), _p(new_counts), C.c_int32(capacity), C.c_uint32(max_count), _p(ws),
+           C.c_size_t(ws.nbytes), C.byref(conv))
+    _check(st, "update_octree")
+    return nn.value, bool(conv.value)
+
+
+def compute_octree(keys, bucket_size, max_count=0xFFFFFFFF, stream=None):
+    """computeOctree (csarray.hpp:429-440) on the device: returns (leaves, counts) as numpy arrays."""
+    kd = keys.dtype
+    root = np.array([0, 1 << (63 if kd == np.uint64 else 30)], dtype=kd)
+    cap = max(1024, 8 * (keys.count // max(bucket_size, 1) + 8))
+    trees = [DeviceArray(cap + 1, kd), DeviceArray(cap + 1, kd)]
+    cnts = [DeviceArray(cap, np.uint32), DeviceArray(cap, np.uint32)]
+    trees[0].upload(root)
+    cnts[0].upload(np.array([keys.count], dtype=np.uint32))
+    n, cur = 1, 0
+    while True:
+        nn, conv = update_octree(keys, bucket_size, trees[cur], cnts[cur], n, trees[1 - cur], cnts[1 - cur], cap,
+                                 max_count, stream)
+        cur, n = 1 - cur, nn
+        if conv:
+            break
+    return trees[cur].download(n + 1), cnts[cur].download(n)
+
+
+def build_octree(leaves):
+    """Link a leaf array (numpy) on the device; returns a dict of numpy arrays (octree.hpp:285-360 layout)."""
+    kd = leaves.dtype
+    L = leaves.size - 1
+    NI = (L - 1) // 7
+    N = L + NI
+    L_max = 21 if kd == np.uint64 else 10
+    d_leaves = DeviceArray.from_numpy(leaves)
+    prefixes = DeviceArray(N, kd)
+    co = DeviceArray(N, np.int32)
+    par = DeviceArray(max(1, (N - 1) // 8), np.int32)
+    lr = DeviceArray(L_max + 2, np.int32)
+    i2l = DeviceArray(N, np.int32)
+    l2i = DeviceArray(N, np.int32)
+    fws = _fn(f"hipcstone_build_octree_workspace_size_{_KS[kd]}")
+    fws.restype = C.c_size_t
+    ws = DeviceArray(fws(C.c_int32(L)), np.uint8)
+    lrh = (C.c_int32 * (L_max + 2))()
+    f = _fn(f"hipcstone_build_octree_{_KS[kd]}")
+    _check(f(None, _p(d_leaves), C.c_int32(L), _p(prefixes), _p(co), _p(par), _p(lr), _p(i2l), _p(l2i), _p(ws),
+             C.c_size_t(ws.nbytes), lrh), "build_octree")
+    out = dict(prefixes=prefixes.download(), childOffsets=co.download(), parents=par.download(),
+               levelRange=lr.download(), internalToLeaf=i2l.download(), leafToInternal=l2i.download())
+    assert np.array_equal(out["levelRange"], np.array(list(lrh), dtype=np.int32))
+    return out
+
+
+def compute_geo_centers(prefixes, box):
+    """Geometric node centers/sizes (source_center_gpu.h:93-99) of a numpy prefix array; returns (centers, sizes)."""
+    T = box_dtype(box)
+    N = prefixes.size
+    dp = DeviceArray.from_numpy(prefixes)
+    c = DeviceArray(3 * N, T)
+    s = DeviceArray(3 * N, T)
+    f = _fn(f"hipcstone_compute_geo_centers_{_KS[prefixes.dtype]}_{_RS[np.dtype(T)]}")
+    _check(f(None, _p(dp), C.c_int32(N), _p(c), _p(s), C.byref(box)), "compute_geo_centers")
+    return c.download().reshape(N, 3), s.download().reshape(N, 3)
+
+
+def leaf_layout(counts, stream=None):
+    """layout = [0, inclusive_scan(counts)] of a device counts array; returns a DeviceArray of L+1."""
+    L = counts.count
+    _lib.hipcstone_scan_workspace_size.restype = C.c_size_t
+    ws = DeviceArray(_lib.hipcstone_scan_workspace_size(C.c_size_t(L)), np.uint8)
+    out = DeviceArray(L + 1, np.uint32)
+    _check(_lib.hipcstone_leaf_layout(_stream(stream), _p(counts), C.c_int32(L), _p(out), _p(ws),
+                                      C.c_size_t(ws.nbytes)), "leaf_layout")
+    return out
+
+
+def compute_bounding_box(x, y, z, h, layout, first, last, scale, centers, sizes, stream=None):
+    T = x.dtype
+    ctype = C.c_float if T == np.float32 else C.c_double
+    f = _fn(f"hipcstone_compute_bounding_box_{_RS[T]}")
+    _check(f(_stream(stream), _p(x), _p(y), _p(z), _p(h), _p(layout), C.c_int32(first), C.c_int32(last),
+             ctype(scale), _p(centers), _p(sizes)), "compute_bounding_box")
+
+
+def find_halos(prefixes, child_offsets, parents, centers, sizes, leaves, search_centers, search_sizes, box, first,
+               last, num_nodes, flags, stream=None):
+    f = _fn(f"hipcstone_find_halos_{_KS[prefixes.dtype]}_{_RS[centers.dtype]}")
+    _check(f(_stream(stream), _p(prefixes), _p(child_offsets), _p(parents), _p(centers), _p(sizes), _p(leaves),
+             _p(search_centers), _p(search_sizes), C.byref(box), C.c_int32(first), C.c_int32(last),
+             C.c_int32(num_nodes), _p(flags)), "find_halos")
+
+
+def find_neighbors(x, y, z, h, first, last, box, child_offsets, parents, internal_to_leaf, layout, centers, sizes,
+                   num_nodes, ngmax, neighbors, counts, ext=1.0, stream=None):
+    """CPU-semantics radius search on the device (findneighbors.hpp:156-177)."""
+    T = x.dtype if isinstance(x, DeviceArray) else np.dtype(box_dtype(box))
+    f = _fn(f"hipcstone_find_neighbors_{_RS[np.dtype(T)]}")
+    _check(f(_stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_uint32(first), C.c_uint32(last), C.byref(box),
+             _p(child_offsets), _p(parents), _p(internal_to_leaf), _p(layout), _p(centers), _p(sizes),
+             C.c_int32(num_nodes), C.c_float(ext), C.c_uint32(ngmax), _p(neighbors), _p(counts)), "find_neighbors")
+
+
+def displace(x, y, z, h, n, scale, seed, step, box, stream=None):
+    """Synthetic multi-step motion: x += scale*h*u, u uniform in [-1,1)^3 (counter-based), clamped into the box."""
+    if isinstance(box, BoxF32):
+        _check(_lib.hipcstone_synth_displace_f32(_stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_size_t(n),
+                                                 C.c_float(scale), C.c_uint64(seed), C.c_uint64(step), C.byref(box)),
+               "synth_displace")
+    else:
+        _check(_lib.hipcstone_synth_displace_f64(_stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_size_t(n),
+                                                 C.c_double(scale), C.c_uint64(seed), C.c_uint64(step), C.byref(box)),
+               "synth_displace")
+
+
+# ---------------------------------------------------------------- pipeline
+
+
+class PipelineView(C.Structure):
+    _fields_ = [
+        ("num_particles", C.c_size_t),
+        ("num_leaves", C.c_int32),
+        ("num_nodes", C.c_int32),
+        ("num_internal", C.c_int32),
+        ("converged_calls", C.c_int32),
+    ] + [(name, C.c_void_p) for name in (
+        "keys", "perm", "x", "y", "z", "h", "leaves", "counts", "prefixes", "child_offsets", "parents",
+        "internal_to_leaf", "leaf_to_internal", "level_range", "centers", "sizes", "layout")] + [
+        ("level_range_host", C.c_int32 * 24)]
+
+
+class Pipeline:
+    """Plain single-domain sync step (hipcstone_pipeline_*): keys, sort+gather, tree update, link, centers,
+    layout and neighbors."""
+
+    def __init__(self, curve=HILBERT, key_bits=64, real_bits=32, bucket_size=64, ngmax=150):
+        self.handle = C.c_void_p()
+        self.key_dtype = np.uint64 if key_bits == 64 else np.uint32
+        self.real_dtype = np.float32 if real_bits == 32 else np.float64
+        self.ngmax = ngmax
+        self.max_level = 21 if key_bits == 64 else 10
+        _check(_lib.hipcstone_pipeline_create(C.byref(self.handle), C.c_int(curve), C.c_int(key_bits),
+                                              C.c_int(real_bits), C.c_uint32(bucket_size), C.c_uint32(ngmax)),
+               "pipeline_create")
+
+    def sync(self, x, y, z, h, n, box, full_build=False, neighbors=None, counts=None, stream=None):
+        _check(_lib.hipcstone_pipeline_sync(self.handle, _stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_size_t(n),
+                                            C.byref(box), C.c_int(int(full_build)), _p(neighbors), _p(counts)),
+               "pipeline_sync")
+
+    STAGES = ("keys", "sort_gather", "tree_update", "link", "centers", "layout", "neighbors")
+
+    def set_timing(self, enable=True):
+        _check(_lib.hipcstone_pipeline_set_timing(self.handle, C.c_int(int(enable))), "pipeline_set_timing")
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} since timing was enabled (synchronizes)."""
+        ms = (C.c_double * 7)()
+        cnt = (C.c_int32 * 7)()
+        _check(_lib.hipcstone_pipeline_stage_times(self.handle, ms, cnt), "pipeline_stage_times")
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(self.STAGES)}
+
+    def view(self) -> PipelineView:
+        v = PipelineView()
+        _check(_lib.hipcstone_pipeline_get_view(self.handle, C.byref(v)), "pipeline_get_view")
+        return v
+
+    def arrays(self):
+        """Download the pipeline state as numpy arrays."""
+        v = self.view()
+        n, L, N = v.num_particles, v.num_leaves, v.num_nodes
+        kd, T = self.key_dtype, self.real_dtype
+        return dict(
+            keys=download_ptr(v.keys, n, kd), perm=download_ptr(v.perm, n, np.uint32),
+            x=download_ptr(v.x, n, T), y=download_ptr(v.y, n, T), z=download_ptr(v.z, n, T),
+            h=download_ptr(v.h, n, T), leaves=download_ptr(v.leaves, L + 1, kd),
+            counts=download_ptr(v.counts, L, np.uint32), prefixes=download_ptr(v.prefixes, N, kd),
+            childOffsets=download_ptr(v.child_offsets, N, np.int32),
+            parents=download_ptr(v.parents, max(1, (N - 1) // 8), np.int32),
+            internalToLeaf=download_ptr(v.internal_to_leaf, N, np.int32),
+            leafToInternal=download_ptr(v.leaf_to_internal, N, np.int32),
+            levelRange=np.array(list(v.level_range_host)[: self.max_level + 2], dtype=np.int32),
+            centers=download_ptr(v.centers, 3 * N, T).reshape(N, 3), sizes=download_ptr(v.sizes, 3 * N, T).reshape(N, 3),
+            layout=download_ptr(v.layout, L + 1, np.uint32),
+        )
+
+    def __del__(self):
+        try:
+            _lib.hipcstone_pipeline_destroy(self.handle)
+        except Exception:
+            pass

@@ -1,0 +1,348 @@
+/*
+ * hipcstone — MI355X-native (gfx950) single-domain octree pipeline: SFC keys, stable radix sort with permutation
+ * gather, cornerstone leaf array build/update, internal-tree linking, geometric node centers, per-leaf search boxes,
+ * halo discovery and radius neighbor search.
+ *
+ * C ABI. All pointers are device pointers unless stated otherwise; every call is stream-ordered on the given
+ * stream (NULL = the default stream) and asynchronous unless it returns a host value. No call allocates device
+ * memory unless its name says so: scratch comes in through explicit workspace arguments whose size is returned by
+ * the matching *_workspace_size() query. Calls are re-entrant: no call keeps state in device globals, so
+ * independent calls may run concurrently on different streams.
+ *
+ * Naming: hipcstone_<function>_<key>_<real> where <key> is u32 (10 levels, 30-bit keys) or u64 (21 levels, 63-bit
+ * keys) and <real> is f32 or f64. Each declaration cites the reference interface it replaces
+ * (sekelle/cornerstone-octree, paths relative to include/cstone/).
+ */
+#ifndef HIPCSTONE_H
+#define HIPCSTONE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum
+{
+    HIPCSTONE_OK            = 0,
+    HIPCSTONE_INVALID_ARG   = 1, /* sizes or pointers inconsistent; the reference throws std::runtime_error */
+    HIPCSTONE_OUT_OF_MEMORY = 2, /* a hipcstone_*_alloc call failed */
+    HIPCSTONE_HIP_ERROR     = 3, /* a HIP runtime call failed; see hipcstone_last_error() */
+    HIPCSTONE_CAPACITY      = 4  /* an output buffer is smaller than the result; the required size is returned */
+} hipcstone_status;
+
+/* curve kinds for key encoding (sfc/sfc.hpp:30-40 MortonKey / HilbertKey) */
+enum
+{
+    HIPCSTONE_MORTON  = 0,
+    HIPCSTONE_HILBERT = 1
+};
+
+/* boundary types (sfc/box.hpp:80-86 BoundaryType) */
+enum
+{
+    HIPCSTONE_OPEN       = 0,
+    HIPCSTONE_PERIODIC   = 1,
+    HIPCSTONE_FIXED      = 2,
+    HIPCSTONE_CUBIC_OPEN = 3
+};
+
+/*
+ * Coordinate box, the value layout of sfc/box.hpp:95-175 Box<T>: limits {xmin,xmax,ymin,ymax,zmin,zmax},
+ * lengths (max-min) and inverse lengths (T(1)/(max-min)) evaluated in T, and the three boundary types.
+ * hipcstone_make_box_* fills it exactly as the Box constructor does (box.hpp:100-122).
+ */
+typedef struct
+{
+    float lim[6];
+    float len[3];
+    float ilen[3];
+    uint8_t bnd[3];
+} hipcstone_box_f32;
+
+typedef struct
+{
+    double lim[6];
+    double len[3];
+    double ilen[3];
+    uint8_t bnd[3];
+} hipcstone_box_f64;
+
+typedef void* hipcstone_stream; /* a hipStream_t */
+
+/* ---- runtime: errors, memory, streams, events (cuda/errorcheck.cuh, cuda/device_vector.h equivalents) ---- */
+
+/* text of the last error on the calling host thread (cuda/errorcheck.cuh:15-27 prints and exits instead) */
+const char* hipcstone_last_error(void);
+/* library version and the gfx target it was built for */
+const char* hipcstone_build_info(void);
+
+hipcstone_status hipcstone_device_alloc(void** ptr, size_t bytes);
+hipcstone_status hipcstone_device_free(void* ptr);
+hipcstone_status hipcstone_memcpy_h2d(void* dst, const void* src, size_t bytes, hipcstone_stream s);
+hipcstone_status hipcstone_memcpy_d2h(void* dst, const void* src, size_t bytes, hipcstone_stream s);
+hipcstone_status hipcstone_memcpy_d2d(void* dst, const void* src, size_t bytes, hipcstone_stream s);
+/* fill with a byte value through a kernel (capturable in a hipGraph, unlike hipMemsetAsync) */
+hipcstone_status hipcstone_fill_bytes(void* dst, uint8_t value, size_t bytes, hipcstone_stream s);
+hipcstone_status hipcstone_stream_create(hipcstone_stream* s);
+hipcstone_status hipcstone_stream_destroy(hipcstone_stream s);
+hipcstone_status hipcstone_stream_sync(hipcstone_stream s);
+hipcstone_status hipcstone_device_sync(void);
+hipcstone_status hipcstone_event_create(void** ev);
+hipcstone_status hipcstone_event_destroy(void* ev);
+hipcstone_status hipcstone_event_record(void* ev, hipcstone_stream s);
+hipcstone_status hipcstone_event_elapsed_ms(void* start, void* stop, float* ms);
+/* host-side Box constructor, sfc/box.hpp:100-122 */
+void hipcstone_make_box_f32(float xmin, float xmax, float ymin, float ymax, float zmin, float zmax, int bx, int by,
+                            int bz, hipcstone_box_f32* box);
+void hipcstone_make_box_f64(double xmin, double xmax, double ymin, double ymax, double zmin, double zmax, int bx,
+                            int by, int bz, hipcstone_box_f64* box);
+
+/* ---- a2/a3: SFC keys. Replaces computeSfcKeys(execution::Gpu, x, y, z, keys, n, box), sfc/sfc_gpu.h:24-26 ----
+ * keys[i] = sfc3D(x[i], y[i], z[i], box) (sfc/sfc.hpp:142-179). With honor_remove_key != 0, keys equal to
+ * removeKey (2^30 / 2^63, tree/definitions.h:70-75) are left unchanged, as the reference kernel does
+ * (sfc/sfc_gpu.cu:24-33); with 0 the key array is write-only. */
+#define HIPCSTONE_DECL_KEYS(CURVE, KS, K, RS, T)                                                                       \
+    hipcstone_status hipcstone_compute_sfc_keys_##CURVE##_##KS##_##RS(hipcstone_stream s, const T* x, const T* y,     \
+                                                                      const T* z, K* keys, size_t n,                   \
+                                                                      const hipcstone_box_##RS* box,                   \
+                                                                      int honor_remove_key);
+HIPCSTONE_DECL_KEYS(hilbert, u32, uint32_t, f32, float)
+HIPCSTONE_DECL_KEYS(hilbert, u32, uint32_t, f64, double)
+HIPCSTONE_DECL_KEYS(hilbert, u64, uint64_t, f32, float)
+HIPCSTONE_DECL_KEYS(hilbert, u64, uint64_t, f64, double)
+HIPCSTONE_DECL_KEYS(morton, u32, uint32_t, f32, float)
+HIPCSTONE_DECL_KEYS(morton, u32, uint32_t, f64, double)
+HIPCSTONE_DECL_KEYS(morton, u64, uint64_t, f32, float)
+HIPCSTONE_DECL_KEYS(morton, u64, uint64_t, f64, double)
+
+/* ---- a4: stable key sort with uint32 values. Replaces sortByKeyTempStorage / sortByKey(execution::Gpu, first,
+ * last, values, keyBuf, valueBuf, tmp, tmpBytes), primitives/primitives_gpu.h:119-130 (CUB SortPairs,
+ * primitives_gpu.cu:322-356). Sorts [keys, keys+n) ascending by bits [0, end_bit) in place, carrying values;
+ * ties keep their input order. key_buf/value_buf hold n elements each. end_bit = 0 means all bits of the key
+ * type. values == NULL sorts keys alone. */
+size_t hipcstone_sort_by_key_workspace_size_u32(size_t n);
+size_t hipcstone_sort_by_key_workspace_size_u64(size_t n);
+hipcstone_status hipcstone_sort_by_key_u32(hipcstone_stream s, uint32_t* keys, uint32_t* values, size_t n,
+                                           uint32_t* key_buf, uint32_t* value_buf, void* workspace,
+                                           size_t workspace_bytes, int end_bit);
+hipcstone_status hipcstone_sort_by_key_u64(hipcstone_stream s, uint64_t* keys, uint32_t* values, size_t n,
+                                           uint64_t* key_buf, uint32_t* value_buf, void* workspace,
+                                           size_t workspace_bytes, int end_bit);
+
+/* ---- a4+a5 fused: keys sorted with the permutation iota(0..n) and x,y,z,h gathered by it, the sequence of
+ * sequence() + sortByKey() + gather() x4 (primitives/primitives_gpu.h:100-155, :48-56; domain/domain.hpp:187-188).
+ * keys/perm are sorted in place (perm is write-only: the identity is generated on the fly); x,y,z,h are read from
+ * the *_in arrays and written in sorted order to *_out. h may be NULL (then h_out is ignored). */
+size_t hipcstone_sort_gather_workspace_size_u32(size_t n);
+size_t hipcstone_sort_gather_workspace_size_u64(size_t n);
+#define HIPCSTONE_DECL_SORTGATHER(KS, K, RS, T)                                                                        \
+    hipcstone_status hipcstone_sort_gather_##KS##_##RS(                                                                \
+        hipcstone_stream s, K* keys, uint32_t* perm, size_t n, const T* x_in, const T* y_in, const T* z_in,           \
+        const T* h_in, T* x_out, T* y_out, T* z_out, T* h_out, K* key_buf, uint32_t* value_buf, void* workspace,       \
+        size_t workspace_bytes);
+HIPCSTONE_DECL_SORTGATHER(u32, uint32_t, f32, float)
+HIPCSTONE_DECL_SORTGATHER(u32, uint32_t, f64, double)
+HIPCSTONE_DECL_SORTGATHER(u64, uint64_t, f32, float)
+HIPCSTONE_DECL_SORTGATHER(u64, uint64_t, f64, double)
+
+/* ---- a5: permutation gather dst[i] = src[perm[i]]. Replaces gather(execution::Gpu, ordering, n, src, dst),
+ * primitives/primitives_gpu.h:48-56 ---- */
+hipcstone_status hipcstone_gather_u32(hipcstone_stream s, const uint32_t* perm, size_t n, const uint32_t* src,
+                                      uint32_t* dst);
+hipcstone_status hipcstone_gather_u64(hipcstone_stream s, const uint32_t* perm, size_t n, const uint64_t* src,
+                                      uint64_t* dst);
+
+/* ---- a6: node counts. Replaces computeNodeCountsGpu(exec, tree, counts, numNodes, keys, maxCount,
+ * useCountsAsGuess), tree/csarray_gpu.h:41-48. counts[i] = min(#keys in [tree[i], tree[i+1]), max_count)
+ * (tree/csarray.hpp:181-235); the guess flag only changes the search, never the result. ---- */
+hipcstone_status hipcstone_compute_node_counts_u32(hipcstone_stream s, const uint32_t* tree, uint32_t* counts,
+                                                   int32_t num_nodes, const uint32_t* keys, size_t num_keys,
+                                                   uint32_t max_count, int use_counts_as_guess);
+hipcstone_status hipcstone_compute_node_counts_u64(hipcstone_stream s, const uint64_t* tree, uint32_t* counts,
+                                                   int32_t num_nodes, const uint64_t* keys, size_t num_keys,
+                                                   uint32_t max_count, int use_counts_as_guess);
+
+/* ---- a7: rebalance decision + scan. Replaces computeNodeOpsGpu(exec, tree, numNodes, counts, bucketSize,
+ * nodeOps), tree/csarray_gpu.h:61-67: node_ops[0..num_nodes] receives the exclusive scan of the per-node ops
+ * (tree/csarray.hpp:267-329); *new_num_nodes (host) = node_ops[num_nodes]; *converged (host) = all ops were 1.
+ * Synchronizes the stream. Workspace: hipcstone_node_ops_workspace_size(num_nodes). ---- */
+size_t hipcstone_node_ops_workspace_size(int32_t num_nodes);
+hipcstone_status hipcstone_compute_node_ops_u32(hipcstone_stream s, const uint32_t* tree, int32_t num_nodes,
+                                                const uint32_t* counts, uint32_t bucket_size, int32_t* node_ops,
+                                                void* workspace, size_t workspace_bytes, int32_t* new_num_nodes,
+                                                int* converged);
+hipcstone_status hipcstone_compute_node_ops_u64(hipcstone_stream s, const uint64_t* tree, int32_t num_nodes,
+                                                const uint32_t* counts, uint32_t bucket_size, int32_t* node_ops,
+                                                void* workspace, size_t workspace_bytes, int32_t* new_num_nodes,
+                                                int* converged);
+
+/* ---- a8: rebalance. Replaces rebalanceTreeGpu(exec, tree, numNodes, newNumNodes, nodeOps, newTree),
+ * tree/csarray_gpu.h:69-75: new_tree[0..new_num_nodes] from the scanned ops (tree/csarray.hpp:339-389). ---- */
+hipcstone_status hipcstone_rebalance_tree_u32(hipcstone_stream s, const uint32_t* tree, int32_t num_nodes,
+                                              int32_t new_num_nodes, const int32_t* node_ops, uint32_t* new_tree);
+hipcstone_status hipcstone_rebalance_tree_u64(hipcstone_stream s, const uint64_t* tree, int32_t num_nodes,
+                                              int32_t new_num_nodes, const int32_t* node_ops, uint64_t* new_tree);
+
+/* ---- a9: one updateOctree step. Replaces updateOctreeGpu(exec, keys, bucketSize, tree, counts, tmpTree,
+ * workArray, maxCount), tree/update_gpu.cuh:43-68 (CPU: tree/csarray.hpp:408-426): decision on (tree, counts),
+ * rebalance into new_tree, recount into new_counts. *num_nodes is updated (host); returns HIPCSTONE_CAPACITY with
+ * *num_nodes set to the required size when capacity (in nodes) is too small, leaving tree untouched.
+ * *converged = the input tree's decision was all-unchanged. Synchronizes the stream once.
+ * Workspace: hipcstone_update_octree_workspace_size(num_nodes). ---- */
+size_t hipcstone_update_octree_workspace_size(int32_t num_nodes);
+hipcstone_status hipcstone_update_octree_u32(hipcstone_stream s, const uint32_t* keys, size_t num_keys,
+                                             uint32_t bucket_size, const uint32_t* tree, const uint32_t* counts,
+                                             int32_t* num_nodes, uint32_t* new_tree, uint32_t* new_counts,
+                                             int32_t capacity, uint32_t max_count, void* workspace,
+                                             size_t workspace_bytes, int* converged);
+hipcstone_status hipcstone_update_octree_u64(hipcstone_stream s, const uint64_t* keys, size_t num_keys,
+                                             uint32_t bucket_size, const uint64_t* tree, const uint32_t* counts,
+                                             int32_t* num_nodes, uint64_t* new_tree, uint32_t* new_counts,
+                                             int32_t capacity, uint32_t max_count, void* workspace,
+                                             size_t workspace_bytes, int* converged);
+
+/* ---- a10: internal tree linking. Replaces buildOctreeGpu(exec, cstoneTree, OctreeView), tree/octree_gpu.h:35-45
+ * (CPU buildOctreeCpu, tree/octree.hpp:167-196). With L = num_leaves and I = (L-1)/7, N = L+I:
+ * prefixes[N], child_offsets[N], parents[max(1,(N-1)/8)], level_range[MAXL+2] (device), internal_to_leaf[N],
+ * leaf_to_internal[N]. level_range_host (optional, host) receives a copy of level_range (synchronizes). ---- */
+size_t hipcstone_build_octree_workspace_size_u32(int32_t num_leaves);
+size_t hipcstone_build_octree_workspace_size_u64(int32_t num_leaves);
+hipcstone_status hipcstone_build_octree_u32(hipcstone_stream s, const uint32_t* leaves, int32_t num_leaves,
+                                            uint32_t* prefixes, int32_t* child_offsets, int32_t* parents,
+                                            int32_t* level_range, int32_t* internal_to_leaf,
+                                            int32_t* leaf_to_internal, void* workspace, size_t workspace_bytes,
+                                            int32_t* level_range_host);
+hipcstone_status hipcstone_build_octree_u64(hipcstone_stream s, const uint64_t* leaves, int32_t num_leaves,
+                                            uint64_t* prefixes, int32_t* child_offsets, int32_t* parents,
+                                            int32_t* level_range, int32_t* internal_to_leaf,
+                                            int32_t* leaf_to_internal, void* workspace, size_t workspace_bytes,
+                                            int32_t* level_range_host);
+
+/* ---- a11: geometric node centers and half sizes (Vec3 arrays, 3 T per node). Replaces
+ * computeGeoCentersGpu(exec, prefixes, numNodes, centers, sizes, box), focus/source_center_gpu.h:93-99.
+ * As in the reference (source_center.hpp:145-158 via sfcKey -> SfcKind), node keys are decoded as Hilbert
+ * keys. ---- */
+#define HIPCSTONE_DECL_CENTERS(KS, K, RS, T)                                                                           \
+    hipcstone_status hipcstone_compute_geo_centers_##KS##_##RS(hipcstone_stream s, const K* prefixes,                 \
+                                                               int32_t num_nodes, T* centers, T* sizes,                \
+                                                               const hipcstone_box_##RS* box);
+HIPCSTONE_DECL_CENTERS(u32, uint32_t, f32, float)
+HIPCSTONE_DECL_CENTERS(u32, uint32_t, f64, double)
+HIPCSTONE_DECL_CENTERS(u64, uint64_t, f32, float)
+HIPCSTONE_DECL_CENTERS(u64, uint64_t, f64, double)
+
+/* ---- a12: leaf layout. layout[0] = 0, layout[i+1] = sum(counts[0..i]) (the inclusive scan of
+ * test/unit/neighbors/findneighbors.cpp:67-68 / focus/octree_focus_mpi.hpp:552-554). ---- */
+size_t hipcstone_scan_workspace_size(size_t n);
+hipcstone_status hipcstone_leaf_layout(hipcstone_stream s, const uint32_t* counts, int32_t num_leaves,
+                                       uint32_t* layout, void* workspace, size_t workspace_bytes);
+
+/* ---- a13: per-leaf search boxes. Replaces computeBoundingBoxGpu(exec, x, y, z, h, layout, first, last, scale,
+ * centers, sizes), focus/source_center_gpu.h:40-51 (CPU source_center.hpp:28-43). centers (in: seed, e.g. the
+ * leaf's geometric center; out: box center) and sizes are leaf-indexed Vec3 arrays. ---- */
+#define HIPCSTONE_DECL_BBOX(RS, T)                                                                                     \
+    hipcstone_status hipcstone_compute_bounding_box_##RS(hipcstone_stream s, const T* x, const T* y, const T* z,      \
+                                                         const T* h, const uint32_t* layout, int32_t first,           \
+                                                         int32_t last, T scale, T* centers, T* sizes);
+HIPCSTONE_DECL_BBOX(f32, float)
+HIPCSTONE_DECL_BBOX(f64, double)
+
+/* ---- a14: halo discovery. Replaces findHalosGpu(exec, prefixes, childOffsets, parents, nodeCenters, nodeSizes,
+ * leaves, searchCenters, searchSizes, box, firstNode, lastNode, collisionFlags), traversal/collisions_gpu.h:46-59
+ * (CPU collisions.hpp:68-94). flags (node-indexed) are set to 1 for halo nodes and never cleared. num_nodes is the
+ * size of the linked tree (bounds every walk). ---- */
+#define HIPCSTONE_DECL_HALOS(KS, K, RS, T)                                                                             \
+    hipcstone_status hipcstone_find_halos_##KS##_##RS(                                                                 \
+        hipcstone_stream s, const K* prefixes, const int32_t* child_offsets, const int32_t* parents,                  \
+        const T* node_centers, const T* node_sizes, const K* leaves, const T* search_centers, const T* search_sizes,   \
+        const hipcstone_box_##RS* box, int32_t first_node, int32_t last_node, int32_t num_nodes, uint8_t* flags);
+HIPCSTONE_DECL_HALOS(u32, uint32_t, f32, float)
+HIPCSTONE_DECL_HALOS(u32, uint32_t, f64, double)
+HIPCSTONE_DECL_HALOS(u64, uint64_t, f32, float)
+HIPCSTONE_DECL_HALOS(u64, uint64_t, f64, double)
+
+/* ---- a16: radius neighbor search with the CPU semantics of findNeighbors(x, y, z, h, firstId, lastId, box,
+ * OctreeNsView, ngmax, neighbors, neighborsCount), findneighbors.hpp:156-177 — the reference has no GPU free
+ * function with this contract; its GPU search is GpuAlwaysTraverseNeighborhoodBuilder
+ * (traversal/ijloop/gpu_alwaystraverse.cuh:174-204). For i in [first_id, last_id): row (i-first_id) of neighbors
+ * (stride ngmax) receives the ascending indices j != i with |x_j - x_i|^2 < 4 h_i^2 among the leaves the
+ * particle's own stackless traversal visits; neighbors_count[i-first_id] is the untruncated count.
+ * ext = OctreeNsView::searchExtFactor (tree/octree.hpp:281). neighbors may be NULL when ngmax == 0. ---- */
+#define HIPCSTONE_DECL_NEIGHBORS(RS, T)                                                                                \
+    hipcstone_status hipcstone_find_neighbors_##RS(                                                                    \
+        hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, uint32_t first_id, uint32_t last_id,      \
+        const hipcstone_box_##RS* box, const int32_t* child_offsets, const int32_t* parents,                          \
+        const int32_t* internal_to_leaf, const uint32_t* layout, const T* centers, const T* sizes, int32_t num_nodes,  \
+        float ext, uint32_t ngmax, uint32_t* neighbors, uint32_t* neighbors_count);
+HIPCSTONE_DECL_NEIGHBORS(f32, float)
+HIPCSTONE_DECL_NEIGHBORS(f64, double)
+
+/* ---- plain pipeline: one sync step of the single-domain tree, host orchestration of the entries above.
+ * Replaces the sequence Domain<KeyType,T,Gpu>::sync at numRanks=1 performs on its global tree
+ * (domain/domain.hpp:169-218 -> domain/assignment.hpp:92-144) followed by octreeProperties() + findNeighbors
+ * (domain.hpp:377-392): keys, stable sort + gather of x,y,z,h, computeOctree (first call or full_build) or one
+ * warm-start updateOctree step, link, geometric centers, leaf layout, and findNeighbors over all particles when
+ * neighbors_count != NULL. Inputs are device arrays of n particles; the sorted copies live in the pipeline (see
+ * hipcstone_pipeline_view) and may be passed back as the next step's input. Buffers grow on demand; a step
+ * synchronizes the stream once per updateOctree call and once to read the level range. ---- */
+typedef struct hipcstone_pipeline hipcstone_pipeline;
+
+typedef struct
+{
+    size_t num_particles;
+    int32_t num_leaves;
+    int32_t num_nodes;
+    int32_t num_internal;
+    int32_t converged_calls; /* updateOctree calls of the last full build */
+    void* keys;              /* sorted keys (K) */
+    void* perm;              /* sorted-to-input permutation (u32) */
+    void* x;                 /* sorted coordinates (T) */
+    void* y;
+    void* z;
+    void* h;
+    void* leaves; /* cornerstone leaf keys, num_leaves + 1 (K) */
+    void* counts; /* particles per leaf (u32) */
+    void* prefixes;
+    void* child_offsets;
+    void* parents;
+    void* internal_to_leaf;
+    void* leaf_to_internal;
+    void* level_range;
+    void* centers; /* geometric node centers, Vec3 per node (T) */
+    void* sizes;
+    void* layout; /* first particle of each leaf, num_leaves + 1 (u32) */
+    int32_t level_range_host[24];
+} hipcstone_pipeline_view;
+
+hipcstone_status hipcstone_pipeline_create(hipcstone_pipeline** p, int curve, int key_bits, int real_bits,
+                                           uint32_t bucket_size, uint32_t ngmax);
+hipcstone_status hipcstone_pipeline_destroy(hipcstone_pipeline* p);
+hipcstone_status hipcstone_pipeline_sync(hipcstone_pipeline* p, hipcstone_stream s, const void* x, const void* y,
+                                         const void* z, const void* h, size_t n, const void* box, int full_build,
+                                         uint32_t* neighbors, uint32_t* neighbors_count);
+hipcstone_status hipcstone_pipeline_get_view(const hipcstone_pipeline* p, hipcstone_pipeline_view* v);
+
+/* Per-stage timing of sync steps with HIP events recorded on the step's stream (no extra synchronization while
+ * enabled). Stages: 0 keys, 1 sort+gather, 2 tree update, 3 link, 4 centers, 5 layout, 6 neighbors.
+ * hipcstone_pipeline_stage_times synchronizes, then returns, per stage, the summed milliseconds and the number of
+ * timed launches since timing was enabled (arrays of HIPCSTONE_PIPELINE_STAGES entries). */
+#define HIPCSTONE_PIPELINE_STAGES 7
+hipcstone_status hipcstone_pipeline_set_timing(hipcstone_pipeline* p, int enable);
+hipcstone_status hipcstone_pipeline_stage_times(hipcstone_pipeline* p, double* ms, int32_t* launches);
+
+/* ---- synthetic workload helper (not an octree stage): x += scale*h*u, u uniform in [-1,1)^3 from a counter-based
+ * hash of (seed, step, particle, dimension), clamped into the box; the moving input of multi-step runs ---- */
+hipcstone_status hipcstone_synth_displace_f32(hipcstone_stream s, float* x, float* y, float* z, const float* h,
+                                              size_t n, float scale, uint64_t seed, uint64_t step,
+                                              const hipcstone_box_f32* box);
+hipcstone_status hipcstone_synth_displace_f64(hipcstone_stream s, double* x, double* y, double* z, const double* h,
+                                              size_t n, double scale, uint64_t seed, uint64_t step,
+                                              const hipcstone_box_f64* box);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPCSTONE_H */

@@ -1,0 +1,264 @@
+"""ORACLE — test infrastructure, not product code.
+
+ctypes access to the two CPU checkers:
+  * ``ORACLE`` — oracle/liboracle.so, the C restatement of the reference's algorithms (oracle.c);
+  * ``REF``    — oracle/_ref/libcstone_ref.so, the reference's own headers compiled from /root/reference by
+                 `make -C oracle ref` (absent where the reference is not mounted; then ``REF`` is None).
+Both expose the same Python methods, so tests can run one check against either. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcstone_ref.so")
+
+HILBERT, MORTON = 1, 0
+
+
+class BoxF32(C.Structure):
+    _fields_ = [("lim", C.c_float * 6), ("len", C.c_float * 3), ("ilen", C.c_float * 3), ("bnd", C.c_uint8 * 3)]
+
+
+class BoxF64(C.Structure):
+    _fields_ = [("lim", C.c_double * 6), ("len", C.c_double * 3), ("ilen", C.c_double * 3), ("bnd", C.c_uint8 * 3)]
+
+
+def make_box(lo, hi, dtype=np.float64, boundary=(0, 0, 0)):
+    """Box<T> with lengths and inverse lengths evaluated in T, as sfc/box.hpp:100-122 does."""
+    T = np.dtype(dtype).type
+    lo = np.broadcast_to(np.asarray(lo, dtype=np.float64), (3,)).astype(T)
+    hi = np.broadcast_to(np.asarray(hi, dtype=np.float64), (3,)).astype(T)
+    b = BoxF32() if T == np.float32 else BoxF64()
+    for d in range(3):
+        b.lim[2 * d] = lo[d]
+        b.lim[2 * d + 1] = hi[d]
+        L = T(hi[d] - lo[d])
+        b.len[d] = L
+        b.ilen[d] = T(T(1.0) / L)
+        b.bnd[d] = int(boundary[d])
+    return b
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else C.c_void_p(0)
+
+
+_KS = {np.dtype(np.uint32): "u32", np.dtype(np.uint64): "u64"}
+_RS = {np.dtype(np.float32): "f32", np.dtype(np.float64): "f64"}
+
+
+class CpuLib:
+    """Common interface over liboracle.so ('orc_' prefix) and libcstone_ref.so ('ref_' prefix)."""
+
+    def __init__(self, path, kind):
+        self.kind = kind
+        self.lib = C.CDLL(path)
+        self.pre = "orc_" if kind == "oracle" else "ref_"
+
+    def _f(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    # ---- keys
+    def compute_sfc_keys(self, x, y, z, box, key_dtype, curve=HILBERT, keys_in=None):
+        x = np.ascontiguousarray(x)
+        kd = np.dtype(key_dtype)
+        keys = np.zeros(x.size, dtype=kd) if keys_in is None else np.array(keys_in, dtype=kd)
+        if self.kind == "oracle":
+            f = self._f(f"compute_sfc_keys_{_KS[kd]}_{_RS[x.dtype]}")
+            f(C.c_int(curve), _ptr(x), _ptr(np.ascontiguousarray(y)), _ptr(np.ascontiguousarray(z)), _ptr(keys),
+              C.c_size_t(x.size), C.byref(box), C.c_int(1))
+        else:
+            f = self._f(f"compute_sfc_keys_{_KS[kd]}")
+            f(C.c_int(curve), _ptr(x), _ptr(np.ascontiguousarray(y)), _ptr(np.ascontiguousarray(z)), _ptr(keys),
+              C.c_size_t(x.size), C.byref(box), C.c_int(int(x.dtype == np.float64)))
+        return keys
+
+    # ---- sort
+    def sort_by_key(self, keys, values=None):
+        k = np.array(keys, copy=True)
+        v = np.arange(k.size, dtype=np.uint32) if values is None else np.array(values, dtype=np.uint32)
+        self._f(f"sort_by_key_{_KS[k.dtype]}")(_ptr(k), _ptr(v), C.c_size_t(k.size))
+        return k, v
+
+    # ---- leaf array
+    def compute_node_counts(self, tree, keys, max_count=0xFFFFFFFF, guess=None):
+        n = tree.size - 1
+        counts = np.zeros(n, dtype=np.uint32) if guess is None else np.array(guess, dtype=np.uint32)
+        self._f(f"compute_node_counts_{_KS[tree.dtype]}")(_ptr(tree), _ptr(counts), C.c_int(n), _ptr(keys),
+                                                           C.c_size_t(keys.size), C.c_uint(max_count),
+                                                           C.c_int(int(guess is not None)))
+        return counts
+
+    def rebalance_decision(self, tree, counts, bucket):
+        n = tree.size - 1
+        ops = np.zeros(n + 1, dtype=np.int32)
+        conv = self._f(f"rebalance_decision_{_KS[tree.dtype]}")(_ptr(tree), _ptr(counts), C.c_int(n),
+                                                                 C.c_uint(bucket), _ptr(ops))
+        return ops[:n], bool(conv)
+
+    def compute_octree(self, keys, bucket, max_count=0xFFFFFFFF):
+        if self.kind == "oracle":
+            tp = C.c_void_p()
+            cp = C.c_void_p()
+            nn = C.c_int()
+            calls = self._f(f"compute_octree_{_KS[keys.dtype]}")(_ptr(keys), C.c_size_t(keys.size),
+                                                                  C.c_uint(bucket), C.byref(tp), C.byref(nn),
+                                                                  C.byref(cp), C.c_uint(max_count))
+            tree = np.ctypeslib.as_array(C.cast(tp, C.POINTER(C.c_uint64 if keys.dtype == np.uint64 else C.c_uint32)),
+                                         shape=(nn.value + 1,)).copy()
+            counts = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_uint32)), shape=(nn.value,)).copy()
+            self.lib.orc_free(tp)
+            self.lib.orc_free(cp)
+            return tree, counts
+        cap = keys.size + 64
+        while True:
+            tree = np.zeros(cap + 1, dtype=keys.dtype)
+            counts = np.zeros(cap, dtype=np.uint32)
+            nn = self._f(f"compute_octree_{_KS[keys.dtype]}")(_ptr(keys), C.c_size_t(keys.size), C.c_uint(bucket),
+                                                               _ptr(tree), C.c_int(cap), _ptr(counts))
+            if nn <= cap:
+                return tree[: nn + 1].copy(), counts[:nn].copy()
+            cap = nn
+
+    def update_octree(self, keys, bucket, tree, counts, max_count=0xFFFFFFFF):
+        """One updateOctree step; returns (tree, counts, converged)."""
+        if self.kind == "oracle":
+            n = tree.size - 1
+            # the oracle replaces the malloc'd tree/counts it is given (orc_update_octree_*)
+            libc = C.CDLL(None)
+            libc.malloc.restype = C.c_void_p
+            tbuf = libc.malloc(C.c_size_t(tree.nbytes))
+            cbuf = libc.malloc(C.c_size_t(max(counts.nbytes, 4)))
+            C.memmove(tbuf, tree.ctypes.data, tree.nbytes)
+            C.memmove(cbuf, counts.ctypes.data, counts.nbytes)
+            tpp = C.c_void_p(tbuf)
+            cpp = C.c_void_p(cbuf)
+            nn = C.c_int(n)
+            conv = self._f(f"update_octree_{_KS[keys.dtype]}")(_ptr(keys), C.c_size_t(keys.size), C.c_uint(bucket),
+                                                                C.byref(tpp), C.byref(nn), C.byref(cpp),
+                                                                C.c_uint(max_count))
+            newt = np.ctypeslib.as_array(C.cast(tpp, C.POINTER(C.c_uint64 if keys.dtype == np.uint64 else C.c_uint32)),
+                                         shape=(nn.value + 1,)).copy()
+            newc = np.ctypeslib.as_array(C.cast(cpp, C.POINTER(C.c_uint32)), shape=(nn.value,)).copy()
+            self.lib.orc_free(tpp)
+            self.lib.orc_free(cpp)
+            return newt, newc, bool(conv)
+        cap = max(tree.size * 4096, 64)
+        t = np.zeros(cap + 1, dtype=tree.dtype)
+        c = np.zeros(cap, dtype=np.uint32)
+        t[: tree.size] = tree
+        c[: counts.size] = counts
+        nn = C.c_int(tree.size - 1)
+        conv = self._f(f"update_octree_{_KS[keys.dtype]}")(_ptr(keys), C.c_size_t(keys.size), C.c_uint(bucket),
+                                                            _ptr(t), C.byref(nn), _ptr(c), C.c_int(cap))
+        return t[: nn.value + 1].copy(), c[: nn.value].copy(), bool(conv)
+
+    # ---- link
+    def build_octree(self, leaves):
+        L = leaves.size - 1
+        NI = (L - 1) // 7
+        N = L + NI
+        Lmax = 21 if leaves.dtype == np.uint64 else 10
+        out = dict(prefixes=np.zeros(N, dtype=leaves.dtype), childOffsets=np.zeros(N, dtype=np.int32),
+                   parents=np.zeros(max(1, (N - 1) // 8), dtype=np.int32), levelRange=np.zeros(Lmax + 2, np.int32),
+                   internalToLeaf=np.zeros(N, np.int32), leafToInternal=np.zeros(N, np.int32))
+        self._f(f"build_octree_{_KS[leaves.dtype]}")(_ptr(leaves), C.c_int(L), _ptr(out["prefixes"]),
+                                                      _ptr(out["childOffsets"]), _ptr(out["parents"]),
+                                                      _ptr(out["levelRange"]), _ptr(out["internalToLeaf"]),
+                                                      _ptr(out["leafToInternal"]))
+        return out
+
+    # ---- geometry
+    def node_fp_centers(self, prefixes, box, real_dtype, curve=HILBERT):
+        T = np.dtype(real_dtype)
+        N = prefixes.size
+        c = np.zeros((N, 3), dtype=T)
+        s = np.zeros((N, 3), dtype=T)
+        f = self._f(f"node_fp_centers_{_KS[prefixes.dtype]}_{_RS[T]}")
+        f(C.c_int(curve), _ptr(prefixes), C.c_size_t(N), _ptr(c), _ptr(s), C.byref(box))
+        return c, s
+
+    def compute_bounding_box(self, x, y, z, h, layout, first, last, scale, centers):
+        T = x.dtype
+        c = np.array(centers, dtype=T, copy=True)
+        s = np.zeros_like(c)
+        ct = C.c_float if T == np.float32 else C.c_double
+        self._f(f"compute_bounding_box_{_RS[T]}")(_ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(layout), C.c_int(first),
+                                                   C.c_int(last), ct(scale), _ptr(c), _ptr(s))
+        return c, s
+
+    def find_halos(self, prefixes, childOffsets, parents, centers, sizes, leaves, sc, ss, box, first, last):
+        flags = np.zeros(prefixes.size, dtype=np.uint8)
+        f = self._f(f"find_halos_{_KS[prefixes.dtype]}_{_RS[centers.dtype]}")
+        f(_ptr(prefixes), _ptr(childOffsets), _ptr(parents), _ptr(np.ascontiguousarray(centers)),
+          _ptr(np.ascontiguousarray(sizes)), _ptr(leaves), _ptr(np.ascontiguousarray(sc)),
+          _ptr(np.ascontiguousarray(ss)), C.byref(box), C.c_int(first), C.c_int(last), _ptr(flags))
+        return flags
+
+    def find_neighbors(self, x, y, z, h, first, last, box, tree, layout, centers, sizes, ngmax, ext=1.0):
+        T = x.dtype
+        n = last - first
+        nb = np.zeros(max(1, n * ngmax), dtype=np.uint32)
+        nc = np.zeros(max(1, n), dtype=np.uint32)
+        f = self._f(f"find_neighbors_{_RS[T]}")
+        f(_ptr(x), _ptr(y), _ptr(z), _ptr(h), C.c_uint32(first), C.c_uint32(last), C.byref(box),
+          _ptr(tree["childOffsets"]), _ptr(tree["parents"]), _ptr(tree["internalToLeaf"]), _ptr(layout),
+          _ptr(np.ascontiguousarray(centers)), _ptr(np.ascontiguousarray(sizes)), C.c_float(ext), C.c_uint(ngmax),
+          _ptr(nb), _ptr(nc))
+        return nb[: n * ngmax].reshape(n, ngmax) if ngmax else nb[:0].reshape(n, 0), nc[:n]
+
+    def find_neighbors_subset(self, x, y, z, h, ids, box, tree, layout, centers, sizes, ngmax, ext=1.0):
+        """Per-particle search for a list of indices (oracle only)."""
+        assert self.kind == "oracle"
+        T = x.dtype
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        nb = np.zeros(max(1, ids.size * ngmax), dtype=np.uint32)
+        nc = np.zeros(max(1, ids.size), dtype=np.uint32)
+        f = self._f(f"find_neighbors_subset_{_RS[T]}")
+        f(_ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(ids), C.c_size_t(ids.size), C.byref(box),
+          _ptr(tree["childOffsets"]), _ptr(tree["parents"]), _ptr(tree["internalToLeaf"]), _ptr(layout),
+          _ptr(np.ascontiguousarray(centers)), _ptr(np.ascontiguousarray(sizes)), C.c_float(ext), C.c_uint(ngmax),
+          _ptr(nb), _ptr(nc))
+        return nb[: ids.size * ngmax].reshape(ids.size, ngmax), nc[: ids.size]
+
+    # ---- reference pipeline timing (REF only)
+    def pipeline(self, x, y, z, h, box, bucket, ngmax, nb_stride=1, tree=None, counts=None, curve=HILBERT,
+                 key_dtype=np.uint64):
+        """One reference CPU pipeline step (keys, stable sort + gather, computeOctree or one warm updateOctree,
+        link, layout, centers, neighbors of every nb_stride-th particle); x,y,z,h are reordered in place.
+        Returns (tree, counts, stage_times_ms[keys, sort, tree, link, centers, neighbors, total], counts_sampled)."""
+        assert self.kind == "ref"
+        T = x.dtype
+        n = x.size
+        cap = max(8 * (n // max(bucket, 1) + 64), 1024)
+        t = np.zeros(cap + 1, dtype=key_dtype)
+        c = np.zeros(cap, dtype=np.uint32)
+        nn = C.c_int(0)
+        if tree is not None:
+            t[: tree.size] = tree
+            c[: counts.size] = counts
+            nn = C.c_int(tree.size - 1)
+        m = (n + nb_stride - 1) // nb_stride
+        nb = np.zeros(max(1, m * ngmax), dtype=np.uint32)
+        nc = np.zeros(max(1, m), dtype=np.uint32)
+        times = np.zeros(7, dtype=np.float64)
+        f = self._f(f"pipeline_{_KS[np.dtype(key_dtype)]}_{_RS[T]}")
+        f(C.c_int(curve), _ptr(x), _ptr(y), _ptr(z), _ptr(h), C.c_size_t(n), C.byref(box), C.c_uint(bucket), _ptr(t),
+          C.byref(nn), _ptr(c), C.c_int(cap), C.c_uint(ngmax), C.c_uint32(nb_stride), _ptr(nb), _ptr(nc),
+          _ptr(times))
+        return t[: nn.value + 1].copy(), c[: nn.value].copy(), times, nc[:m]
+
+    def omp_threads(self):
+        assert self.kind == "ref"
+        return int(self.lib.ref_omp_threads())
+
+
+ORACLE = CpuLib(ORACLE_PATH, "oracle") if os.path.exists(ORACLE_PATH) else None
+REF = CpuLib(REF_PATH, "ref") if os.path.exists(REF_PATH) else None

@@ -1,0 +1,354 @@
+/*
+ * ORACLE — test infrastructure, not product code.
+ *
+ * C-callable wrappers around the reference's own CPU functions. This file contains no reference code: it
+ * includes the reference headers where they lie (-I /root/reference/include, see oracle/Makefile) and is built
+ * into oracle/_ref/libcstone_ref.so, which only tests/ and bench.py's cpu_baseline leg load, to pin the
+ * oracle's C restatement and to time the reference CPU path. Argument layouts match oracle.c.
+ */
+#include <chrono>
+#include <omp.h>
+#include <cstdint>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "cstone/findneighbors.hpp"
+#include "cstone/focus/source_center.hpp"
+#include "cstone/primitives/gather.hpp"
+#include "cstone/sfc/sfc.hpp"
+#include "cstone/traversal/collisions.hpp"
+#include "cstone/tree/csarray.hpp"
+#include "cstone/tree/octree.hpp"
+
+using namespace cstone;
+
+namespace
+{
+
+template<class T>
+struct BoxC
+{
+    T lim[6];
+    T len[3];
+    T ilen[3];
+    uint8_t bnd[3];
+};
+
+template<class T>
+Box<T> toBox(const BoxC<T>* b)
+{
+    return Box<T>(b->lim[0], b->lim[1], b->lim[2], b->lim[3], b->lim[4], b->lim[5], BoundaryType(b->bnd[0]),
+                  BoundaryType(b->bnd[1]), BoundaryType(b->bnd[2]));
+}
+
+template<class T>
+const Vec3<T>* v3(const T* p)
+{
+    return reinterpret_cast<const Vec3<T>*>(p);
+}
+
+template<class T>
+Vec3<T>* v3(T* p)
+{
+    return reinterpret_cast<Vec3<T>*>(p);
+}
+
+template<class K>
+void keysImpl(int curve, const void* x, const void* y, const void* z, K* keys, size_t n, const void* box, int isDouble)
+{
+    if (isDouble)
+    {
+        auto b = toBox(static_cast<const BoxC<double>*>(box));
+        if (curve == 1)
+            computeSfcKeys((const double*)x, (const double*)y, (const double*)z, (HilbertKey<K>*)keys, n, b);
+        else
+            computeSfcKeys((const double*)x, (const double*)y, (const double*)z, (MortonKey<K>*)keys, n, b);
+    }
+    else
+    {
+        auto b = toBox(static_cast<const BoxC<float>*>(box));
+        if (curve == 1)
+            computeSfcKeys((const float*)x, (const float*)y, (const float*)z, (HilbertKey<K>*)keys, n, b);
+        else
+            computeSfcKeys((const float*)x, (const float*)y, (const float*)z, (MortonKey<K>*)keys, n, b);
+    }
+}
+
+template<class K>
+int computeOctreeImpl(const K* keys, size_t n, unsigned bucket, K* treeOut, int cap, unsigned* countsOut)
+{
+    auto [tree, counts] = computeOctree<K>(std::span<const K>(keys, n), bucket);
+    int nn              = int(nNodes(tree));
+    if (nn <= cap)
+    {
+        std::copy(tree.begin(), tree.end(), treeOut);
+        std::copy(counts.begin(), counts.end(), countsOut);
+    }
+    return nn;
+}
+
+template<class K>
+int updateOctreeImpl(const K* keys, size_t n, unsigned bucket, K* tree, int* numNodes, unsigned* counts, int cap)
+{
+    std::vector<K> t(tree, tree + *numNodes + 1);
+    std::vector<unsigned> c(counts, counts + *numNodes);
+    bool conv = updateOctree<K>(std::span<const K>(keys, n), bucket, t, c);
+    int nn    = int(nNodes(t));
+    if (nn <= cap)
+    {
+        std::copy(t.begin(), t.end(), tree);
+        std::copy(c.begin(), c.end(), counts);
+    }
+    *numNodes = nn;
+    return conv ? 1 : 0;
+}
+
+template<class K>
+void buildOctreeImpl(const K* leaves, int numLeaves, K* prefixes, int32_t* childOffsets, int32_t* parents,
+                     int32_t* levelRange, int32_t* internalToLeaf, int32_t* leafToInternal)
+{
+    int numInternal = (numLeaves - 1) / 7;
+    buildOctreeCpu(leaves, numLeaves, numInternal, prefixes, childOffsets, parents, levelRange, internalToLeaf,
+                   leafToInternal);
+}
+
+template<class T>
+void findNeighborsImpl(const T* x, const T* y, const T* z, const T* h, uint32_t first, uint32_t last,
+                       const BoxC<T>* box, const int32_t* childOffsets, const int32_t* parents,
+                       const int32_t* internalToLeaf, const uint32_t* layout, const T* centers, const T* sizes,
+                       float ext, unsigned ngmax, uint32_t* nb, uint32_t* nc)
+{
+    OctreeNsView<T, uint64_t> v{};
+    v.childOffsets    = childOffsets;
+    v.parents         = parents;
+    v.internalToLeaf  = internalToLeaf;
+    v.layout          = layout;
+    v.centers         = v3(centers);
+    v.sizes           = v3(sizes);
+    v.searchExtFactor = ext;
+    findNeighbors(x, y, z, h, first, last, toBox(box), v, ngmax, nb, nc);
+}
+
+template<class K, class T>
+void findHalosImpl(const K* prefixes, const int32_t* childOffsets, const int32_t* parents, const T* nodeCenters,
+                   const T* nodeSizes, const K* leaves, const T* searchCenters, const T* searchSizes,
+                   const BoxC<T>* box, int32_t firstNode, int32_t lastNode, uint8_t* flags)
+{
+    findHalos(prefixes, childOffsets, parents, v3(nodeCenters), v3(nodeSizes), leaves, v3(searchCenters),
+              v3(searchSizes), toBox(box), firstNode, lastNode, flags);
+}
+
+template<class K, class T>
+void centersImpl(int /*curve: the reference decodes node keys as Hilbert keys (sfcKey -> SfcKind)*/, const K* prefixes,
+                 size_t n, T* centers, T* sizes, const BoxC<T>* box)
+{
+    nodeFpCenters<K>(std::span<const K>(prefixes, n), v3(centers), v3(sizes), toBox(box));
+}
+
+template<class T>
+void bboxImpl(const T* x, const T* y, const T* z, const T* h, const uint32_t* layout, int32_t first, int32_t last,
+              T scale, T* centers, T* sizes)
+{
+#pragma omp parallel for
+    for (int32_t leaf = first; leaf < last; ++leaf)
+    {
+        auto [c, s] = computeBoundingBox(x, y, z, h, layout[leaf], layout[leaf + 1], scale, v3(centers)[leaf]);
+        v3(centers)[leaf] = c;
+        v3(sizes)[leaf]   = s;
+    }
+}
+
+double nowMs()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+/*
+ * One step of the plain pipeline as the reference's CPU path runs it (the sequence of
+ * test/unit/neighbors/findneighbors.cpp:43-93 and test/performance/octree.cpp:29-100):
+ * keys, stable sort with iota, gather x,y,z,h, updateOctree (warm start from tree/counts, or computeOctree when
+ * numNodes == 0), updateInternalTree, layout scan, nodeFpCenters, then the reference's per-particle findNeighbors
+ * (findneighbors.hpp:77-154) for every nbStride-th particle, in an OpenMP loop like the batch version (:156-177).
+ * Arrays x,y,z,h are reordered in place. Stage times (ms) go to times[0..6].
+ */
+template<class K, class T>
+int pipelineImpl(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>* boxc, unsigned bucket, K* tree,
+                 int* numNodes, unsigned* counts, int cap, unsigned ngmax, uint32_t nbStride, uint32_t* nb,
+                 uint32_t* nc, double* times)
+{
+    Box<T> box = toBox(boxc);
+    std::vector<K> keys(n);
+    double t0 = nowMs();
+    if (curve == 1) computeSfcKeys(x, y, z, (HilbertKey<K>*)keys.data(), n, box);
+    else computeSfcKeys(x, y, z, (MortonKey<K>*)keys.data(), n, box);
+    double t1 = nowMs();
+    std::vector<LocalIndex> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    sort_by_key(keys.begin(), keys.end(), perm.begin());
+    std::vector<T> tmp(n);
+    for (T* a : {x, y, z, h})
+    {
+        gather<LocalIndex>(std::span<const LocalIndex>(perm.data(), n), a, tmp.data());
+        std::copy(tmp.begin(), tmp.end(), a);
+    }
+    double t2 = nowMs();
+    std::vector<K> t;
+    std::vector<unsigned> c;
+    if (*numNodes == 0)
+    {
+        auto [tt, cc] = computeOctree<K>(std::span<const K>(keys.data(), n), bucket);
+        t             = std::move(tt);
+        c             = std::move(cc);
+    }
+    else
+    {
+        t.assign(tree, tree + *numNodes + 1);
+        c.assign(counts, counts + *numNodes);
+        updateOctree<K>(std::span<const K>(keys.data(), n), bucket, t, c);
+    }
+    double t3 = nowMs();
+    OctreeData<K, execution::Cpu> octree;
+    octree.resize(nNodes(t));
+    updateInternalTree<K>(t, octree.data());
+    double t4 = nowMs();
+    std::vector<LocalIndex> layout(nNodes(t) + 1);
+    layout[0] = 0;
+    std::inclusive_scan(c.begin(), c.end(), layout.begin() + 1);
+    std::vector<Vec3<T>> centers(octree.numNodes), sizes(octree.numNodes);
+    nodeFpCenters<K>(std::span<const K>(octree.prefixes.data(), octree.numNodes), centers.data(), sizes.data(), box);
+    double t5 = nowMs();
+    OctreeNsView<T, K> v{};
+    v.numLeafNodes   = octree.numLeafNodes;
+    v.numNodes       = octree.numNodes;
+    v.prefixes       = octree.prefixes.data();
+    v.childOffsets   = octree.childOffsets.data();
+    v.parents        = octree.parents.data();
+    v.internalToLeaf = octree.internalToLeaf.data();
+    v.leafToInternal = octree.leafToInternal.data();
+    v.levelRange     = octree.levelRange.data();
+    v.leaves         = t.data();
+    v.layout         = layout.data();
+    v.centers        = centers.data();
+    v.sizes          = sizes.data();
+    int64_t numTargets = (int64_t(n) + nbStride - 1) / nbStride;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < numTargets; ++k)
+    {
+        LocalIndex i = LocalIndex(k * nbStride);
+        nc[k]        = findNeighbors(i, x, y, z, h, v, box, ngmax, nb + size_t(k) * ngmax);
+    }
+    double t6 = nowMs();
+    times[0]  = t1 - t0;
+    times[1]  = t2 - t1;
+    times[2]  = t3 - t2;
+    times[3]  = t4 - t3;
+    times[4]  = t5 - t4;
+    times[5]  = t6 - t5;
+    times[6]  = t6 - t0;
+    int nn    = int(nNodes(t));
+    if (nn <= cap)
+    {
+        std::copy(t.begin(), t.end(), tree);
+        std::copy(c.begin(), c.end(), counts);
+    }
+    *numNodes = nn;
+    return nn;
+}
+
+} // namespace
+
+extern "C"
+{
+
+#define REF_KEYS(K, KS)                                                                                                \
+    void ref_compute_sfc_keys_##KS(int curve, const void* x, const void* y, const void* z, K* keys, size_t n,         \
+                                   const void* box, int isDouble)                                                      \
+    {                                                                                                                  \
+        keysImpl<K>(curve, x, y, z, keys, n, box, isDouble);                                                           \
+    }                                                                                                                  \
+    void ref_sort_by_key_##KS(K* keys, uint32_t* values, size_t n) { sort_by_key(keys, keys + n, values); }            \
+    void ref_compute_node_counts_##KS(const K* tree, unsigned* counts, int nNodes, const K* keys, size_t nKeys,       \
+                                      unsigned maxCount, int useGuess)                                                 \
+    {                                                                                                                  \
+        computeNodeCounts(tree, counts, nNodes, std::span<const K>(keys, nKeys), maxCount, useGuess != 0);           \
+    }                                                                                                                  \
+    int ref_rebalance_decision_##KS(const K* tree, const unsigned* counts, int nNodes, unsigned bucket, int32_t* ops)  \
+    {                                                                                                                  \
+        return rebalanceDecision(tree, counts, nNodes, bucket, ops) ? 1 : 0;                                           \
+    }                                                                                                                  \
+    int ref_compute_octree_##KS(const K* keys, size_t n, unsigned bucket, K* tree, int cap, unsigned* counts)         \
+    {                                                                                                                  \
+        return computeOctreeImpl<K>(keys, n, bucket, tree, cap, counts);                                              \
+    }                                                                                                                  \
+    int ref_update_octree_##KS(const K* keys, size_t n, unsigned bucket, K* tree, int* numNodes, unsigned* counts,   \
+                               int cap)                                                                                \
+    {                                                                                                                  \
+        return updateOctreeImpl<K>(keys, n, bucket, tree, numNodes, counts, cap);                                     \
+    }                                                                                                                  \
+    void ref_build_octree_##KS(const K* leaves, int numLeaves, K* prefixes, int32_t* childOffsets, int32_t* parents,  \
+                               int32_t* levelRange, int32_t* internalToLeaf, int32_t* leafToInternal)                  \
+    {                                                                                                                  \
+        buildOctreeImpl<K>(leaves, numLeaves, prefixes, childOffsets, parents, levelRange, internalToLeaf,            \
+                           leafToInternal);                                                                            \
+    }
+
+    REF_KEYS(uint32_t, u32)
+    REF_KEYS(uint64_t, u64)
+
+#define REF_KR(K, KS, T, TS)                                                                                           \
+    void ref_node_fp_centers_##KS##_##TS(int curve, const K* prefixes, size_t n, T* centers, T* sizes,               \
+                                         const BoxC<T>* box)                                                           \
+    {                                                                                                                  \
+        centersImpl<K, T>(curve, prefixes, n, centers, sizes, box);                                                    \
+    }                                                                                                                  \
+    void ref_find_halos_##KS##_##TS(const K* prefixes, const int32_t* childOffsets, const int32_t* parents,           \
+                                    const T* nodeCenters, const T* nodeSizes, const K* leaves, const T* searchCenters,\
+                                    const T* searchSizes, const BoxC<T>* box, int32_t firstNode, int32_t lastNode,     \
+                                    uint8_t* flags)                                                                    \
+    {                                                                                                                  \
+        findHalosImpl<K, T>(prefixes, childOffsets, parents, nodeCenters, nodeSizes, leaves, searchCenters,           \
+                            searchSizes, box, firstNode, lastNode, flags);                                             \
+    }                                                                                                                  \
+    int ref_pipeline_##KS##_##TS(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>* box, unsigned bucket,   \
+                                 K* tree, int* numNodes, unsigned* counts, int cap, unsigned ngmax,                    \
+                                 uint32_t nbStride, uint32_t* nb, uint32_t* nc, double* times)                         \
+    {                                                                                                                  \
+        return pipelineImpl<K, T>(curve, x, y, z, h, n, box, bucket, tree, numNodes, counts, cap, ngmax,              \
+                                  nbStride, nb, nc, times);                                                            \
+    }
+
+    REF_KR(uint32_t, u32, float, f32)
+    REF_KR(uint32_t, u32, double, f64)
+    REF_KR(uint64_t, u64, float, f32)
+    REF_KR(uint64_t, u64, double, f64)
+
+#define REF_T(T, TS)                                                                                                   \
+    void ref_find_neighbors_##TS(const T* x, const T* y, const T* z, const T* h, uint32_t first, uint32_t last,       \
+                                 const BoxC<T>* box, const int32_t* childOffsets, const int32_t* parents,              \
+                                 const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,              \
+                                 const T* sizes, float ext, unsigned ngmax, uint32_t* nb, uint32_t* nc)                \
+    {                                                                                                                  \
+        findNeighborsImpl<T>(x, y, z, h, first, last, box, childOffsets, parents, internalToLeaf, layout, centers,    \
+                             sizes, ext, ngmax, nb, nc);                                                               \
+    }                                                                                                                  \
+    void ref_compute_bounding_box_##TS(const T* x, const T* y, const T* z, const T* h, const uint32_t* layout,        \
+                                       int32_t first, int32_t last, T scale, T* centers, T* sizes)                     \
+    {                                                                                                                  \
+        bboxImpl<T>(x, y, z, h, layout, first, last, scale, centers, sizes);                                           \
+    }
+
+    REF_T(float, f32)
+    REF_T(double, f64)
+
+    int ref_omp_threads()
+    {
+        int n = 1;
+#pragma omp parallel
+        {
+#pragma omp single
+            n = omp_get_num_threads();
+        }
+        return n;
+    }
+}

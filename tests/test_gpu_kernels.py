@@ -1,0 +1,304 @@
+"""GPU parity: every C-ABI entry point against the oracle on the same seeded inputs (bit-exact)."""
+
+import numpy as np
+import pytest
+
+from helpers import OctreeMaker, cpu_pipeline, neighbor_rows_equal, uniform_n_level_tree
+from oracle_py import HILBERT, MORTON
+from oracle_py import make_box as orc_box
+
+pytestmark = pytest.mark.gpu
+
+KDS = [np.uint32, np.uint64]
+TS = [np.float32, np.float64]
+
+
+def dev(hip, a):
+    return hip.DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+
+def boxes(hip, T, lo, hi, bnd=(0, 0, 0)):
+    return hip.make_box(lo, hi, T, bnd), orc_box(lo, hi, T, bnd)
+
+
+def plummer_input(n, T, a=0.1, seed=21, target=60):
+    from hipcstone import inputs
+
+    x, y, z, r = inputs.plummer(n, a, seed=seed, dtype=T)
+    h = inputs.plummer_h(r, n, a, target, dtype=T)
+    return x, y, z, h
+
+
+# ---------------------------------------------------------------- keys
+
+
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("curve", [HILBERT, MORTON])
+@pytest.mark.parametrize("T", TS)
+def test_keys(hip, oracle, kd, curve, T):
+    x, y, z, _ = plummer_input(100003, T)
+    for lo, hi in ((-1.0, 1.0), ((-1.0, -1.1, -0.95), (1.05, 1.3, 1.0))):
+        gbox, obox = boxes(hip, T, lo, hi)
+        dx, dy, dz = dev(hip, x), dev(hip, y), dev(hip, z)
+        dk = hip.DeviceArray(x.size, kd)
+        hip.compute_sfc_keys(dx, dy, dz, dk, gbox, curve)
+        assert np.array_equal(dk.download(), oracle.compute_sfc_keys(x, y, z, obox, kd, curve))
+
+
+def test_keys_remove_key(hip, oracle):
+    T, kd = np.float64, np.uint64
+    x = np.linspace(0.01, 0.99, 1000)
+    keys_in = np.zeros(1000, dtype=kd)
+    keys_in[::7] = 1 << 63
+    gbox, obox = boxes(hip, T, 0.0, 1.0)
+    dk = dev(hip, keys_in)
+    hip.compute_sfc_keys(dev(hip, x), dev(hip, x), dev(hip, x), dk, gbox, HILBERT, honor_remove_key=True)
+    assert np.array_equal(dk.download(), oracle.compute_sfc_keys(x, x, x, obox, kd, HILBERT, keys_in=keys_in))
+
+
+# ---------------------------------------------------------------- sort
+
+
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("n", [1, 2, 63, 4095, 4096, 4097, 100000, 1 << 20])
+def test_sort_by_key_stable(hip, kd, n):
+    rng = np.random.default_rng(n)
+    hi = 1 << (30 if kd == np.uint32 else 63)
+    keys = rng.integers(0, hi, size=n, dtype=np.uint64).astype(kd)
+    keys[::3] = keys[0]  # duplicates pin the tie order
+    vals = rng.permutation(n).astype(np.uint32)
+    dk, dv = dev(hip, keys), dev(hip, vals)
+    hip.sort_by_key(dk, dv)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(dk.download(), keys[order])
+    assert np.array_equal(dv.download(), vals[order])
+
+
+@pytest.mark.parametrize("end_bit", [8, 17, 30])
+def test_sort_partial_bits_and_keys_only(hip, end_bit):
+    rng = np.random.default_rng(end_bit)
+    keys = rng.integers(0, 1 << end_bit, size=50000, dtype=np.uint64).astype(np.uint32)
+    dk = dev(hip, keys)
+    hip.sort_by_key(dk, None, end_bit=end_bit)
+    assert np.array_equal(dk.download(), np.sort(keys))
+
+
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("T", TS)
+def test_sort_gather(hip, oracle, kd, T):
+    x, y, z, h = plummer_input(300001, T)
+    gbox, obox = boxes(hip, T, -1.0, 1.0)
+    keys = oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT)
+    ks, perm = oracle.sort_by_key(keys)
+    dk = dev(hip, keys)
+    dp = hip.DeviceArray(x.size, np.uint32)
+    outs = [hip.DeviceArray(x.size, T) for _ in range(4)]
+    hip.sort_gather(dk, dp, dev(hip, x), dev(hip, y), dev(hip, z), dev(hip, h), *outs)
+    assert np.array_equal(dk.download(), ks)
+    assert np.array_equal(dp.download(), perm)
+    for o, a in zip(outs, (x, y, z, h)):
+        assert np.array_equal(o.download(), a[perm])
+
+
+# ---------------------------------------------------------------- leaf array
+
+
+@pytest.mark.parametrize("kd", KDS)
+def test_node_counts(hip, oracle, kd):
+    tree = OctreeMaker(kd).divide().divide(0).divide(3).divide(0, 4).make_tree()
+    rng = np.random.default_rng(7)
+    keys = np.sort(rng.integers(0, int(tree[-1]), size=100000, dtype=np.uint64).astype(kd))
+    dc = hip.DeviceArray(tree.size - 1, np.uint32)
+    hip.compute_node_counts(dev(hip, tree), dc, tree.size - 1, dev(hip, keys))
+    assert np.array_equal(dc.download(), oracle.compute_node_counts(tree, keys))
+    hip.compute_node_counts(dev(hip, tree), dc, tree.size - 1, dev(hip, keys), max_count=100)
+    assert np.array_equal(dc.download(), oracle.compute_node_counts(tree, keys, max_count=100))
+
+
+@pytest.mark.parametrize("kd", KDS)
+def test_node_ops_and_rebalance_kat(hip, oracle, kd):
+    # csarray.cpp:108-122 / GPU csarray.cu:85-129: scanned ops {0,1,1,...}
+    tree = OctreeMaker(kd).divide().divide(0).make_tree()
+    counts = np.array([1, 1, 1, 0, 0, 0, 0, 0, 2, 3, 4, 5, 6, 7, 8], dtype=np.uint32)
+    ops = hip.DeviceArray(16, np.int32)
+    new_n, conv = hip.compute_node_ops(dev(hip, tree), 15, dev(hip, counts), 4, ops)
+    ref_ops, ref_conv = oracle.rebalance_decision(tree, counts, 4)
+    scanned = np.concatenate([[0], np.cumsum(ref_ops)]).astype(np.int32)
+    assert np.array_equal(ops.download(), scanned) and new_n == scanned[-1] and conv == ref_conv
+    new_tree = hip.DeviceArray(new_n + 1, kd)
+    hip.rebalance_tree(dev(hip, tree), 15, new_n, ops, new_tree)
+    t_ref, _, _ = oracle.update_octree(np.zeros(0, dtype=kd), 4, tree, counts)
+    assert np.array_equal(new_tree.download(), t_ref)
+
+
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("bucket", [1, 16, 64, 128])
+def test_compute_octree(hip, oracle, kd, bucket):
+    x, y, z, _ = plummer_input(200000, np.float64, a=0.05)
+    _, obox = boxes(hip, np.float64, -1.0, 1.0)
+    keys = np.sort(oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT))
+    t_gpu, c_gpu = hip.compute_octree(dev(hip, keys), bucket)
+    t_ref, c_ref = oracle.compute_octree(keys, bucket)
+    assert np.array_equal(t_gpu, t_ref) and np.array_equal(c_gpu, c_ref)
+
+
+@pytest.mark.parametrize("kd", KDS)
+def test_update_octree_warm_steps(hip, oracle, kd):
+    """Exactly one updateOctree per step from the previous leaves and counts (csarray.hpp:408-426)."""
+    x, y, z, _ = plummer_input(150000, np.float64, a=0.05)
+    _, obox = boxes(hip, np.float64, -1.0, 1.0)
+    keys = np.sort(oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT))
+    tree, counts = oracle.compute_octree(keys, 64)
+    rng = np.random.default_rng(3)
+    for step in range(4):
+        x = np.clip(x + rng.normal(0, 0.01, x.size), -1, 1)
+        keys = np.sort(oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT))
+        cap = tree.size * 64
+        dt, dc = dev(hip, tree), dev(hip, counts)
+        nt, nc = hip.DeviceArray(cap + 1, kd), hip.DeviceArray(cap, np.uint32)
+        nn, conv = hip.update_octree(dev(hip, keys), 64, dt, dc, tree.size - 1, nt, nc, cap)
+        rt, rc, rconv = oracle.update_octree(keys, 64, tree, counts)
+        assert np.array_equal(nt.download(nn + 1), rt) and np.array_equal(nc.download(nn), rc) and conv == rconv
+        tree, counts = rt, rc
+
+
+# ---------------------------------------------------------------- link, centers, layout
+
+
+@pytest.mark.parametrize("kd", KDS)
+def test_build_octree(hip, oracle, kd):
+    cases = [uniform_n_level_tree(kd, 2), OctreeMaker(kd).divide().divide(0).make_tree(),
+             OctreeMaker(kd).make_tree()]
+    x, y, z, _ = plummer_input(300000, np.float64, a=0.05)
+    _, obox = boxes(hip, np.float64, -1.0, 1.0)
+    keys = np.sort(oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT))
+    cases.append(oracle.compute_octree(keys, 16)[0])
+    for tree in cases:
+        g = hip.build_octree(tree)
+        r = oracle.build_octree(tree)
+        N = r["prefixes"].size
+        for k in ("prefixes", "childOffsets", "levelRange", "internalToLeaf", "leafToInternal"):
+            assert np.array_equal(g[k], r[k]), k
+        if N > 1:
+            assert np.array_equal(g["parents"], r["parents"])
+
+
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("T", TS)
+def test_geo_centers(hip, oracle, kd, T):
+    x, y, z, _ = plummer_input(200000, np.float64, a=0.05)
+    _, obox64 = boxes(hip, np.float64, -1.0, 1.0)
+    keys = np.sort(oracle.compute_sfc_keys(x, y, z, obox64, kd, HILBERT))
+    pf = oracle.build_octree(oracle.compute_octree(keys, 8)[0])["prefixes"]
+    for lo, hi in ((-1.0, 1.0), ((-1.0, -1.1, -0.95), (1.05, 1.3, 1.0))):
+        gbox, obox = boxes(hip, T, lo, hi)
+        c, s = hip.compute_geo_centers(pf, gbox)
+        rc, rs = oracle.node_fp_centers(pf, obox, T)
+        assert np.array_equal(c, rc) and np.array_equal(s, rs)
+
+
+@pytest.mark.parametrize("L", [1, 100, 4096, 70001])
+def test_leaf_layout(hip, L):
+    counts = np.random.default_rng(L).integers(0, 200, size=L).astype(np.uint32)
+    lay = hip.leaf_layout(dev(hip, counts)).download()
+    assert lay[0] == 0 and np.array_equal(lay[1:], np.cumsum(counts, dtype=np.uint64).astype(np.uint32))
+
+
+# ---------------------------------------------------------------- neighbors
+
+
+def _gpu_neighbors(hip, p, gbox, first, last, ngmax, ext=1.0):
+    T = p["x"].dtype
+    d = {k: dev(hip, p[k]) for k in ("x", "y", "z", "h", "childOffsets", "parents", "internalToLeaf", "layout")}
+    dc, ds = dev(hip, p["centers"].reshape(-1)), dev(hip, p["sizes"].reshape(-1))
+    n = last - first
+    nb = hip.DeviceArray(max(1, n * ngmax), np.uint32) if ngmax else None
+    nc = hip.DeviceArray(max(1, n), np.uint32)
+    hip.find_neighbors(d["x"], d["y"], d["z"], d["h"], first, last, gbox, d["childOffsets"], d["parents"],
+                       d["internalToLeaf"], d["layout"], dc, ds, p["prefixes"].size, ngmax, nb, nc, ext)
+    counts = nc.download(n)
+    rows = nb.download(n * ngmax).reshape(n, ngmax) if ngmax else np.zeros((n, 0), np.uint32)
+    return rows, counts
+
+
+@pytest.mark.parametrize("T", TS)
+@pytest.mark.parametrize("kd", KDS)
+def test_find_neighbors(hip, oracle, T, kd):
+    n = 60000
+    x, y, z, h = plummer_input(n, T, a=0.2, target=80)
+    gbox, obox = boxes(hip, T, -1.0, 1.0)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, kd, 64, HILBERT)
+    tree = {k: p[k] for k in ("childOffsets", "parents", "internalToLeaf")}
+    for ngmax in (150, 32):
+        ref_nb, ref_nc = oracle.find_neighbors(p["x"], p["y"], p["z"], p["h"], 0, n, obox, tree, p["layout"],
+                                               p["centers"], p["sizes"], ngmax)
+        nb, nc = _gpu_neighbors(hip, p, gbox, 0, n, ngmax)
+        assert neighbor_rows_equal(nb, nc, ref_nb, ref_nc, ngmax)
+
+
+@pytest.mark.parametrize("boundary", [(1, 1, 1), (1, 0, 0)])
+def test_find_neighbors_pbc_ext_subrange(hip, oracle, boundary):
+    T = np.float64
+    n = 40000
+    x, y, z, h = plummer_input(n, T, a=0.3, target=50)
+    gbox, obox = boxes(hip, T, (-1.0, -1.05, -1.0), (1.0, 1.0, 1.1), boundary)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, np.uint64, 32, HILBERT)
+    tree = {k: p[k] for k in ("childOffsets", "parents", "internalToLeaf")}
+    first, last = 1234, 31000
+    ref_nb, ref_nc = oracle.find_neighbors(p["x"], p["y"], p["z"], p["h"], first, last, obox, tree, p["layout"],
+                                           p["centers"], p["sizes"], 64, 1.25)
+    nb, nc = _gpu_neighbors(hip, p, gbox, first, last, 64, 1.25)
+    assert neighbor_rows_equal(nb, nc, ref_nb, ref_nc, 64)
+    # counts only
+    _, nc0 = _gpu_neighbors(hip, p, gbox, first, last, 0, 1.25)
+    assert np.array_equal(nc0, ref_nc)
+
+
+def test_find_neighbors_coincident_and_tiny(hip, oracle):
+    """Coincident particles are neighbors (only j == i is excluded, findneighbors.hpp:141); n < one wave."""
+    T = np.float32
+    x = np.array([0.5, 0.5, 0.5, 0.25, 0.75, 0.5], dtype=T)
+    y = np.array([0.5, 0.5, 0.5, 0.25, 0.75, 0.51], dtype=T)
+    z = np.array([0.5, 0.5, 0.5, 0.25, 0.75, 0.5], dtype=T)
+    h = np.full(6, 0.05, dtype=T)
+    gbox, obox = boxes(hip, T, 0.0, 1.0)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, np.uint64, 2, HILBERT, ngmax=8)
+    nb, nc = _gpu_neighbors(hip, p, gbox, 0, 6, 8)
+    assert neighbor_rows_equal(nb, nc, p["nb"], p["nc"], 8)
+    assert p["nc"].max() == 3
+
+
+# ---------------------------------------------------------------- search boxes and halos
+
+
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("T", TS)
+def test_bbox_and_halos(hip, oracle, kd, T):
+    from hipcstone import inputs
+
+    n = 120000
+    x, y, z = inputs.gaussian_mixture(n, clusters=16, seed=31, dtype=T)
+    h = np.full(n, 0.004, dtype=T)
+    gbox, obox = boxes(hip, T, 0.0, 1.0)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, kd, 64, MORTON)
+    L = p["counts"].size
+    NI = p["prefixes"].size - L
+    init = p["centers"][p["leafToInternal"][NI:]]
+    first, last = L // 4, L // 2
+    rc, rs = oracle.compute_bounding_box(p["x"], p["y"], p["z"], p["h"], p["layout"], first, last, T(2), init)
+    d = {k: dev(hip, p[k]) for k in ("x", "y", "z", "h", "layout", "prefixes", "childOffsets", "parents", "leaves")}
+    dsc = dev(hip, init.reshape(-1))
+    dss = hip.DeviceArray(3 * L, T)
+    dss.fill(0)
+    hip.compute_bounding_box(d["x"], d["y"], d["z"], d["h"], d["layout"], first, last, T(2), dsc, dss)
+    gc = dsc.download().reshape(L, 3)
+    gs = dss.download().reshape(L, 3)
+    assert np.array_equal(gc[first:last], rc[first:last]) and np.array_equal(gs[first:last], rs[first:last])
+    flags = hip.DeviceArray(p["prefixes"].size, np.uint8)
+    flags.fill(0)
+    hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dev(hip, p["centers"].reshape(-1)),
+                   dev(hip, p["sizes"].reshape(-1)), d["leaves"], dsc, dss, gbox, first, last, p["prefixes"].size,
+                   flags)
+    ref = oracle.find_halos(p["prefixes"], p["childOffsets"], p["parents"], p["centers"], p["sizes"], p["leaves"], rc,
+                            rs, obox, first, last)
+    assert np.array_equal(flags.download(), ref) and ref.sum() > 0
