@@ -301,7 +301,7 @@ __host__ __device__ inline K sfc3D(T x, T y, T z, const Box& box)
 
 // ---- wave64 primitives ----
 
-__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // number of set bits of mask below this lane
 __device__ inline unsigned laneRank(uint64_t mask)

@@ -6,13 +6,19 @@
 // and at each visited leaf records j != i with  dx*dx + dy*dy + dz*dz < 4 h_i^2  (left fold), ascending j,
 // keeping the first ngmax and counting all.
 //
-// GPU design (wave-cooperative stackless walk): one wave64 owns 64 consecutive targets (compact in space because
-// they are SFC-ordered) and walks the tree once, with a wave-uniform node cursor driven by childOffsets/parents.
-// Each lane carries `dead`, the depth at which its own predicate chain last failed, so at every node a lane
-// evaluates its own test only if all its ancestors passed — exactly the CPU's visited set, not the superset a
-// group-box walk would give. The wave descends where any lane passes. At a leaf, coordinates of candidate j are
-// wave-uniform (scalar loads) and every lane tests them against its own particle, which also keeps the per-lane
-// neighbor order ascending. A step cap (2 * numNodes + 8) bounds the walk even for a malformed tree.
+// GPU design — one wave64 owns 64 consecutive targets (compact in space because they are SFC-ordered):
+//  * Walk: depth-first with an explicit per-wave stack in LDS. Descending into a node evaluates its 8 children at
+//    once (their centers/sizes are contiguous, so a few wide scalar loads fetch them) and stores, per child, the
+//    ballot of lanes whose own predicate chain passed: a lane is tested at a child only if it passed the parent.
+//    The visited leaf set of every lane is therefore exactly the CPU's, not the superset a group-box walk gives.
+//    Children are taken in order 0..7, so leaves come out in the CPU's visiting order (ascending SFC order).
+//  * Leaves and their lane masks go to a per-wave list, drained in visiting order, so every target's neighbors come
+//    out in ascending j.
+//  * Leaf phase in pair slots: lanes are (target slot, leaf particle) pairs, so nearly all 64 lanes do useful pair
+//    tests even when only a few targets visit a leaf (a per-target lane layout leaves two thirds of the lanes idle
+//    on typical leaves). A ballot ranks each hit among its target's hits in ascending j, so every target's hits of a
+//    leaf are stored as one contiguous run of its row.
+//  * Every loop is bounded: the walk by 2 * numNodes + 8 steps and the stack depth by the key's level count.
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -20,22 +26,63 @@ namespace hipcstone
 {
 
 constexpr int kNbThreads = 256;
-constexpr int kDeadNever = 1 << 30;
+constexpr int kNbWaves   = kNbThreads / kWave;
+constexpr int kMaxDepth  = 22; // 21 levels below the root for 64-bit keys
+constexpr int kLeafList  = 64;
+
+template<class T>
+struct alignas(4 * sizeof(T)) Particle4
+{
+    T x, y, z, w;
+};
+
+template<class T>
+struct NbShared
+{
+    uint64_t mask[kNbWaves][kMaxDepth][8];
+    int32_t co[kNbWaves][kMaxDepth];
+    int32_t next[kNbWaves][kMaxDepth];
+    int32_t listLeaf[kNbWaves][kLeafList];
+    uint64_t listMask[kNbWaves][kLeafList];
+    Particle4<T> target[kNbWaves][kWave]; // x, y, z, 4h^2 of each lane's particle
+    uint64_t rowOff[kNbWaves][kWave];     // element offset of each lane's output row
+    uint32_t count[kNbWaves][kWave];      // neighbors found so far, per target lane
+    uint8_t pbc[kNbWaves][kWave];         // periodic-image flags per target lane
+    uint8_t active[kNbWaves][2 * kWave];  // compacted active target lanes of the current leaf
+};
+
+// order LDS writes of some lanes before LDS reads of other lanes of the same wave
+__device__ inline void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint64_t uniform64(uint64_t v)
+{
+    uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
+    uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
+    return (uint64_t(hi) << 32) | lo;
+}
 
 template<class T, bool AnyPbc>
 __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, const T* __restrict__ h,
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
-    const int32_t* __restrict__ parents, const int32_t* __restrict__ internalToLeaf,
-    const uint32_t* __restrict__ layout, const T* __restrict__ centers, const T* __restrict__ sizes, int32_t numNodes,
-    float ext, uint32_t ngmax, uint32_t* __restrict__ neighbors, uint32_t* __restrict__ neighborsCount)
+    const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
+    const T* __restrict__ sizes, int32_t numNodes, float ext, uint32_t ngmax, uint32_t* __restrict__ neighbors,
+    uint32_t* __restrict__ neighborsCount)
 {
-    const int lane        = laneId();
-    const uint32_t group  = uniform(uint32_t((size_t(blockIdx.x) * kNbThreads + threadIdx.x) / kWave));
-    const uint64_t first  = uint64_t(firstId) + uint64_t(group) * kWave;
+    __shared__ NbShared<T> sh;
+    const int lane       = laneId();
+    const int w          = threadIdx.x / kWave;
+    const uint32_t group = uniform(uint32_t((size_t(blockIdx.x) * kNbThreads + threadIdx.x) / kWave));
+    const uint64_t first = uint64_t(firstId) + uint64_t(group) * kWave;
     if (first >= lastId) return;
-    const uint32_t i = uint32_t(first) + uint32_t(lane);
-    const bool valid = i < lastId;
+    const uint32_t i     = uint32_t(first) + uint32_t(lane);
+    const bool valid     = i < lastId;
+    const size_t rowBase = size_t(group) * kWave; // row of lane 0 in the output
 
     T xi = 0, yi = 0, zi = 0, hi = 0;
     if (valid)
@@ -64,93 +111,184 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         else { return d; }
     };
 
-    auto nodePasses = [&](int32_t node) -> bool {
-        const T* c = centers + size_t(node) * 3;
-        const T* s = sizes + size_t(node) * 3;
-        T d0       = fabs(fold(c[0] - xi, px, 0)) - s[0];
-        T d1       = fabs(fold(c[1] - yi, py, 1)) - s[1];
-        T d2       = fabs(fold(c[2] - zi, pz, 2)) - s[2];
+    // minDistance + right-fold norm2 against a node given by its center and half size
+    auto nodeDist2 = [&](T c0, T c1, T c2, T s0, T s1, T s2) -> T {
+        T d0 = fabs(fold(c0 - xi, px, 0)) - s0;
+        T d1 = fabs(fold(c1 - yi, py, 1)) - s1;
+        T d2 = fabs(fold(c2 - zi, pz, 2)) - s2;
         d0 += fabs(d0);
         d1 += fabs(d1);
         d2 += fabs(d2);
         d0 *= T(0.5);
         d1 *= T(0.5);
         d2 *= T(0.5);
-        return (d0 * d0 + (d1 * d1 + d2 * d2)) < cellR2;
+        return d0 * d0 + (d1 * d1 + d2 * d2);
     };
 
-    uint32_t count = 0;
-    uint32_t* row  = neighbors ? neighbors + size_t(i - firstId) * ngmax : nullptr;
+    // per-target state in LDS, read by whichever lanes test that target
+    sh.target[w][lane]  = Particle4<T>{xi, yi, zi, radiusSq};
+    sh.rowOff[w][lane]  = uint64_t(rowBase + lane) * ngmax;
+    sh.count[w][lane]   = 0;
+    sh.pbc[w][lane]     = uint8_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0));
+    waveSync();
 
-    auto leafAction = [&](int32_t node, bool act) {
-        int32_t leafIdx = uniform(internalToLeaf[node]);
-        uint32_t jb     = uniform(layout[leafIdx]);
-        uint32_t je     = uniform(layout[leafIdx + 1]);
-        for (uint32_t j = jb; j < je; ++j)
+    // ---- leaf phase. A leaf chunk of nj <= 64 particles is tested against its active targets in pair slots:
+    //      lane = (slot, jj) with slot = lane / nj, jj = lane % nj, S = 64 / nj slots; slot s of iteration it tests
+    //      active target k = it*S + s against particle jj. Within a slot, lanes are ascending jj, so the ballot of
+    //      hits ranks each hit among its target's hits in ascending j and all of a target's hits of the chunk land
+    //      contiguously in its row (coalesced runs). The slot's last lane advances the target's count. ----
+    int listCount = 0;
+    auto drain = [&]() {
+        for (int e = 0; e < listCount; ++e)
         {
-            T xx = fold(x[j] - xi, px, 0);
-            T yy = fold(y[j] - yi, py, 1);
-            T zz = fold(z[j] - zi, pz, 2);
-            bool hit = act && j != i && (xx * xx + yy * yy + zz * zz < radiusSq);
-            if (hit)
+            int32_t leafIdx = uniform(sh.listLeaf[w][e]);
+            uint64_t m      = uniform64(sh.listMask[w][e]);
+            uint32_t jb     = uniform(layout[leafIdx]);
+            uint32_t je     = uniform(layout[leafIdx + 1]);
+            int pc          = __builtin_popcountll(m);
+            if ((m >> lane) & 1u) sh.active[w][laneRank(m)] = uint8_t(lane);
+            waveSync();
+            for (uint32_t cs = jb; cs < je; cs += kWave)
             {
-                if (count < ngmax) row[count] = j;
-                count++;
+                uint32_t nj      = je - cs < uint32_t(kWave) ? je - cs : uint32_t(kWave);
+                uint32_t magic   = (65536u + nj - 1) / nj; // floor(l / nj) == (l * magic) >> 16 for l < 64
+                uint32_t S       = uint32_t(kWave) / nj;
+                uint32_t slot    = (uint32_t(lane) * magic) >> 16;
+                uint32_t jj      = uint32_t(lane) - slot * nj;
+                bool jv          = slot < S;
+                uint32_t j       = cs + jj;
+                uint64_t below   = (uint64_t(1) << (slot * nj)) - 1; // lanes of earlier slots (slot*nj < 64)
+                bool last        = jv && jj == nj - 1;
+                T xj = 0, yj = 0, zj = 0;
+                if (jv)
+                {
+                    xj = x[j];
+                    yj = y[j];
+                    zj = z[j];
+                }
+                for (int k0 = 0; k0 < pc; k0 += int(S))
+                {
+                    int k          = k0 + int(slot);
+                    bool tv        = jv & (k < pc);
+                    int t          = sh.active[w][k < 2 * kWave ? k : 0];
+                    Particle4<T> p = sh.target[w][t];
+                    uint32_t c     = sh.count[w][t];
+                    bool fx = false, fy = false, fz = false;
+                    if constexpr (AnyPbc)
+                    {
+                        uint8_t f = sh.pbc[w][t];
+                        fx        = f & 1;
+                        fy        = f & 2;
+                        fz        = f & 4;
+                    }
+                    T xx            = fold(xj - p.x, fx, 0);
+                    T yy            = fold(yj - p.y, fy, 1);
+                    T zz            = fold(zj - p.z, fz, 2);
+                    bool hit        = tv & (xx * xx + yy * yy + zz * zz < p.w) & (j != uint32_t(first) + uint32_t(t));
+                    uint64_t H      = ballot(hit);
+                    uint32_t rank   = laneRank(H & ~below);
+                    uint32_t pos    = c + rank;
+                    if (hit & (pos < ngmax))
+                    {
+                        neighbors[sh.rowOff[w][t] + pos] = j;
+                    }
+                    if (last & tv) sh.count[w][t] = pos + (hit ? 1u : 0u);
+                }
+                waveSync();
             }
         }
+        listCount = 0;
     };
 
-    int dead = valid ? kDeadNever : -1;
+    auto appendLeaf = [&](int32_t node, uint64_t m) {
+        int32_t leafIdx = uniform(internalToLeaf[node]);
+        if (lane == 0)
+        {
+            sh.listLeaf[w][listCount] = leafIdx;
+            sh.listMask[w][listCount] = m;
+        }
+        waveSync();
+        listCount++;
+        if (listCount == kLeafList) drain();
+    };
+
+    // evaluate the 8 children of a node for the lanes in `pm` and push them as a stack frame; the 48 node values
+    // are loaded up front (contiguous, wave-uniform) and the tests are branch-free
+    auto pushFrame = [&](int depth, int32_t co, uint64_t pm) {
+        const bool eligible = (pm >> lane) & 1u;
+        const T* c          = centers + size_t(co) * 3;
+        const T* s          = sizes + size_t(co) * 3;
+        T cv[24], sv[24];
+#pragma unroll
+        for (int k = 0; k < 24; ++k)
+        {
+            cv[k] = c[k];
+            sv[k] = s[k];
+        }
+        uint64_t mk[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            T d2  = nodeDist2(cv[3 * k], cv[3 * k + 1], cv[3 * k + 2], sv[3 * k], sv[3 * k + 1], sv[3 * k + 2]);
+            mk[k] = ballot(eligible & (d2 < cellR2));
+        }
+        if (lane == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                sh.mask[w][depth][k] = mk[k];
+            sh.co[w][depth]   = co;
+            sh.next[w][depth] = 0;
+        }
+        waveSync();
+    };
 
     // root (traversal.hpp:35-45)
-    bool pass = valid && nodePasses(0);
-    if (ballot(pass) != 0)
+    uint64_t rootMask = ballot(valid & (nodeDist2(centers[0], centers[1], centers[2], sizes[0], sizes[1], sizes[2]) <
+                                        cellR2));
+    if (rootMask != 0)
     {
         int32_t rootChild = uniform(childOffsets[0]);
-        if (rootChild == 0) { leafAction(0, pass); }
+        if (rootChild == 0) { appendLeaf(0, rootMask); }
         else
         {
-            dead             = pass ? kDeadNever : (valid ? 0 : -1);
-            int32_t node     = rootChild;
-            int depth        = 1;
-            bool backtrack   = false;
+            int depth = 0;
+            pushFrame(0, rootChild, rootMask);
             int64_t stepsLeft = 2 * int64_t(numNodes) + 8;
-            while (node != 0 && stepsLeft-- > 0)
+            while (depth >= 0 && stepsLeft-- > 0)
             {
-                if (node < 0 || node >= numNodes) break;
-                int32_t co  = uniform(childOffsets[node]);
-                bool isLeaf = co == 0;
-                bool descend = false;
-                if (!backtrack)
+                int k      = uniform(sh.next[w][depth]);
+                uint64_t m = 0;
+                while (k < 8)
                 {
-                    bool eligible = dead >= depth;
-                    bool t        = eligible && nodePasses(node);
-                    if (eligible) dead = t ? kDeadNever : depth;
-                    descend = ballot(t) != 0;
-                    if (isLeaf && descend) leafAction(node, t);
+                    m = uniform64(sh.mask[w][depth][k]);
+                    if (m != 0) break;
+                    ++k;
                 }
-                int32_t sib = (node - 1) % 8;
-                if (!isLeaf && descend)
+                if (k >= 8)
                 {
-                    node      = co;
-                    depth     = depth + 1;
-                    backtrack = false;
+                    --depth;
+                    continue;
                 }
-                else if (sib < 7)
-                {
-                    node      = node + 1;
-                    backtrack = false;
-                }
+                int32_t node = uniform(sh.co[w][depth]) + k;
+                if (lane == 0) sh.next[w][depth] = k + 1;
+                waveSync();
+                if (node <= 0 || node >= numNodes) break; // malformed tree guard
+                int32_t cco = uniform(childOffsets[node]);
+                if (cco == 0) { appendLeaf(node, m); }
                 else
                 {
-                    node      = uniform(parents[(node - 1) / 8]);
-                    depth     = depth - 1;
-                    backtrack = true;
+                    if (depth + 1 >= kMaxDepth || cco <= node || cco + 8 > numNodes) break;
+                    ++depth;
+                    pushFrame(depth, cco, m);
                 }
             }
         }
+        drain();
     }
-    if (valid) neighborsCount[i - firstId] = count;
+
+    waveSync();
+    if (valid) neighborsCount[i - firstId] = sh.count[w][lane];
 }
 
 template<class T>
@@ -174,12 +312,12 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
                   box->bnd[2] == HIPCSTONE_PERIODIC;
     if (anyPbc)
         findNeighborsKernel<T, true><<<blocks, kNbThreads, 0, stream(s)>>>(
-            x, y, z, h, firstId, lastId, *box, childOffsets, parents, internalToLeaf, layout, centers, sizes, numNodes,
-            ext, ngmax, neighbors, neighborsCount);
+            x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
+            ngmax, neighbors, neighborsCount);
     else
         findNeighborsKernel<T, false><<<blocks, kNbThreads, 0, stream(s)>>>(
-            x, y, z, h, firstId, lastId, *box, childOffsets, parents, internalToLeaf, layout, centers, sizes, numNodes,
-            ext, ngmax, neighbors, neighborsCount);
+            x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
+            ngmax, neighbors, neighborsCount);
     HC_CHECK_LAUNCH("findNeighborsKernel");
     return HIPCSTONE_OK;
 }
