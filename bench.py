@@ -100,7 +100,26 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (reference reading for HBM traffic counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import hipcstone as hip
+
+        nbytes = 1 << 30
+        src = hip.DeviceArray(nbytes, np.uint8)
+        dst = hip.DeviceArray(nbytes, np.uint8)
+        src.fill(1)
+        ev = [hip.Event() for _ in range(4)]
+        ev[0].record()
+        for k in range(3):
+            hip._check(hip._lib.hipcstone_memcpy_d2d(hip._p(dst), hip._p(src), hip.C.c_size_t(nbytes), None), "copy")
+            ev[k + 1].record()
+        hip.device_sync()
+        ms = [ev[k].elapsed_ms(ev[k + 1]) for k in range(3)]
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms, "GBs": [2 * nbytes / (t / 1e3) / 1e9 for t in ms]}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "

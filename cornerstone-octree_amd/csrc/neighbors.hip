@@ -19,6 +19,8 @@
 //    on typical leaves). A ballot ranks each hit among its target's hits in ascending j, so every target's hits of a
 //    leaf are stored as one contiguous run of its row.
 //  * Every loop is bounded: the walk by 2 * numNodes + 8 steps and the stack depth by the key's level count.
+#include <type_traits>
+
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -30,10 +32,50 @@ constexpr int kNbWaves   = kNbThreads / kWave;
 constexpr int kMaxDepth  = 22; // 21 levels below the root for 64-bit keys
 constexpr int kLeafList  = 64;
 
+// per leaf-chunk size nj in [1, 64]: slots = 64 / nj and the multiplier with floor(l / nj) == (l * magic) >> 16
+// for every lane l < 64 (checked exhaustively), so the per-chunk slot layout needs no division
+struct SlotTables
+{
+    uint32_t magic[kWave + 1];
+    uint32_t slots[kWave + 1];
+};
+
+constexpr SlotTables makeSlotTables()
+{
+    SlotTables t{};
+    for (int nj = 1; nj <= kWave; ++nj)
+    {
+        t.magic[nj] = (65536u + uint32_t(nj) - 1) / uint32_t(nj);
+        t.slots[nj] = uint32_t(kWave / nj);
+    }
+    return t;
+}
+
+__constant__ SlotTables kSlotTables = makeSlotTables();
+
+// XCD-aware work order: workgroups are dealt round-robin to the 8 XCDs (each with its own L2); remap so that each
+// XCD walks one contiguous range of groups, whose neighbouring targets share leaves and tree nodes in its L2
+constexpr uint32_t kXcds = 8;
+
+__device__ inline uint32_t xcdContiguousBlock(uint32_t b, uint32_t numBlocksPadded)
+{
+    uint32_t perXcd = numBlocksPadded / kXcds;
+    return (b % kXcds) * perXcd + b / kXcds;
+}
+
 template<class T>
 struct alignas(4 * sizeof(T)) Particle4
 {
     T x, y, z, w;
+};
+
+template<class T>
+struct alignas(4 * sizeof(T)) TargetRec
+{
+    Particle4<T> p; // x, y, z, 4h^2
+    uint32_t count; // neighbors found so far
+    uint32_t pbc;   // periodic-image flags
+    uint64_t rowOff; // element offset of the output row
 };
 
 template<class T>
@@ -44,11 +86,8 @@ struct NbShared
     int32_t next[kNbWaves][kMaxDepth];
     int32_t listLeaf[kNbWaves][kLeafList];
     uint64_t listMask[kNbWaves][kLeafList];
-    Particle4<T> target[kNbWaves][kWave]; // x, y, z, 4h^2 of each lane's particle
-    uint64_t rowOff[kNbWaves][kWave];     // element offset of each lane's output row
-    uint32_t count[kNbWaves][kWave];      // neighbors found so far, per target lane
-    uint8_t pbc[kNbWaves][kWave];         // periodic-image flags per target lane
-    uint8_t active[kNbWaves][2 * kWave];  // compacted active target lanes of the current leaf
+    TargetRec<T> target[kNbWaves][kWave]; // per lane: x, y, z, 4h^2, neighbors so far, pbc flags, row offset
+    uint8_t active[kNbWaves][2 * kWave];  // compacted active target lanes of the current leaf (+ zero padding)
 };
 
 // order LDS writes of some lanes before LDS reads of other lanes of the same wave
@@ -72,12 +111,13 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
     const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
     const T* __restrict__ sizes, int32_t numNodes, float ext, uint32_t ngmax, uint32_t* __restrict__ neighbors,
-    uint32_t* __restrict__ neighborsCount)
+    uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded)
 {
     __shared__ NbShared<T> sh;
     const int lane       = laneId();
     const int w          = threadIdx.x / kWave;
-    const uint32_t group = uniform(uint32_t((size_t(blockIdx.x) * kNbThreads + threadIdx.x) / kWave));
+    const uint32_t block = xcdContiguousBlock(blockIdx.x, numBlocksPadded);
+    const uint32_t group = uniform(block * uint32_t(kNbWaves) + uint32_t(w));
     const uint64_t first = uint64_t(firstId) + uint64_t(group) * kWave;
     if (first >= lastId) return;
     const uint32_t i     = uint32_t(first) + uint32_t(lane);
@@ -113,23 +153,20 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
 
     // minDistance + right-fold norm2 against a node given by its center and half size
     auto nodeDist2 = [&](T c0, T c1, T c2, T s0, T s1, T s2) -> T {
-        T d0 = fabs(fold(c0 - xi, px, 0)) - s0;
-        T d1 = fabs(fold(c1 - yi, py, 1)) - s1;
-        T d2 = fabs(fold(c2 - zi, pz, 2)) - s2;
-        d0 += fabs(d0);
-        d1 += fabs(d1);
-        d2 += fabs(d2);
-        d0 *= T(0.5);
-        d1 *= T(0.5);
-        d2 *= T(0.5);
+        // the reference's d += |d|; d *= 0.5 is exactly max(d, +0) for finite d (2d and the halving are exact,
+        // and d + |d| is +0 for d <= 0)
+        T d0 = fmax(fabs(fold(c0 - xi, px, 0)) - s0, T(0));
+        T d1 = fmax(fabs(fold(c1 - yi, py, 1)) - s1, T(0));
+        T d2 = fmax(fabs(fold(c2 - zi, pz, 2)) - s2, T(0));
         return d0 * d0 + (d1 * d1 + d2 * d2);
     };
 
     // per-target state in LDS, read by whichever lanes test that target
-    sh.target[w][lane]  = Particle4<T>{xi, yi, zi, radiusSq};
-    sh.rowOff[w][lane]  = uint64_t(rowBase + lane) * ngmax;
-    sh.count[w][lane]   = 0;
-    sh.pbc[w][lane]     = uint8_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0));
+    sh.target[w][lane] = TargetRec<T>{Particle4<T>{xi, yi, zi, radiusSq}, 0u,
+                                      uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)),
+                                      uint64_t(rowBase + lane) * ngmax};
+    sh.active[w][lane]         = 0;
+    sh.active[w][lane + kWave] = 0;
     waveSync();
 
     // ---- leaf phase. A leaf chunk of nj <= 64 particles is tested against its active targets in pair slots:
@@ -150,15 +187,15 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
             waveSync();
             for (uint32_t cs = jb; cs < je; cs += kWave)
             {
-                uint32_t nj      = je - cs < uint32_t(kWave) ? je - cs : uint32_t(kWave);
-                uint32_t magic   = (65536u + nj - 1) / nj; // floor(l / nj) == (l * magic) >> 16 for l < 64
-                uint32_t S       = uint32_t(kWave) / nj;
-                uint32_t slot    = (uint32_t(lane) * magic) >> 16;
-                uint32_t jj      = uint32_t(lane) - slot * nj;
-                bool jv          = slot < S;
-                uint32_t j       = cs + jj;
-                uint64_t below   = (uint64_t(1) << (slot * nj)) - 1; // lanes of earlier slots (slot*nj < 64)
-                bool last        = jv && jj == nj - 1;
+                uint32_t nj    = je - cs < uint32_t(kWave) ? je - cs : uint32_t(kWave);
+                uint32_t magic = uniform(kSlotTables.magic[nj]);
+                uint32_t S     = uniform(kSlotTables.slots[nj]);
+                uint32_t slot  = (uint32_t(lane) * magic) >> 16;
+                uint32_t jj    = uint32_t(lane) - slot * nj;
+                bool jv        = slot < S;
+                uint32_t j     = cs + jj;
+                uint64_t below = (uint64_t(1) << (slot * nj)) - 1; // lanes of earlier slots (slot*nj < 64)
+                bool last      = jv && jj == nj - 1;
                 T xj = 0, yj = 0, zj = 0;
                 if (jv)
                 {
@@ -166,34 +203,37 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                     yj = y[j];
                     zj = z[j];
                 }
-                for (int k0 = 0; k0 < pc; k0 += int(S))
-                {
-                    int k          = k0 + int(slot);
-                    bool tv        = jv & (k < pc);
-                    int t          = sh.active[w][k < 2 * kWave ? k : 0];
-                    Particle4<T> p = sh.target[w][t];
-                    uint32_t c     = sh.count[w][t];
-                    bool fx = false, fy = false, fz = false;
-                    if constexpr (AnyPbc)
+                // j == i is possible only if the chunk overlaps this wave's own targets
+                const bool selfChunk = cs < uint32_t(first) + uint32_t(kWave) && cs + nj > uint32_t(first);
+                const uint32_t jRel  = j - uint32_t(first);
+
+                auto iterate = [&](auto checkSelf) {
+                    for (int k0 = 0; k0 < pc; k0 += int(S))
                     {
-                        uint8_t f = sh.pbc[w][t];
-                        fx        = f & 1;
-                        fy        = f & 2;
-                        fz        = f & 4;
+                        int k              = k0 + int(slot); // < 128: pc <= 64 and slot < S <= 64
+                        bool tv            = jv & (k < pc);
+                        int t              = sh.active[w][k];
+                        const TargetRec<T> rec = sh.target[w][t];
+                        bool fx = false, fy = false, fz = false;
+                        if constexpr (AnyPbc)
+                        {
+                            fx = rec.pbc & 1;
+                            fy = rec.pbc & 2;
+                            fz = rec.pbc & 4;
+                        }
+                        T xx     = fold(xj - rec.p.x, fx, 0);
+                        T yy     = fold(yj - rec.p.y, fy, 1);
+                        T zz     = fold(zj - rec.p.z, fz, 2);
+                        bool hit = tv & (xx * xx + yy * yy + zz * zz < rec.p.w);
+                        if constexpr (decltype(checkSelf)::value) { hit = hit & (jRel != uint32_t(t)); }
+                        uint64_t H    = ballot(hit);
+                        uint32_t pos  = rec.count + laneRank(H & ~below);
+                        if (hit & (pos < ngmax)) neighbors[rec.rowOff + pos] = j;
+                        if (last & tv) sh.target[w][t].count = pos + (hit ? 1u : 0u);
                     }
-                    T xx            = fold(xj - p.x, fx, 0);
-                    T yy            = fold(yj - p.y, fy, 1);
-                    T zz            = fold(zj - p.z, fz, 2);
-                    bool hit        = tv & (xx * xx + yy * yy + zz * zz < p.w) & (j != uint32_t(first) + uint32_t(t));
-                    uint64_t H      = ballot(hit);
-                    uint32_t rank   = laneRank(H & ~below);
-                    uint32_t pos    = c + rank;
-                    if (hit & (pos < ngmax))
-                    {
-                        neighbors[sh.rowOff[w][t] + pos] = j;
-                    }
-                    if (last & tv) sh.count[w][t] = pos + (hit ? 1u : 0u);
-                }
+                };
+                if (selfChunk) iterate(std::true_type{});
+                else iterate(std::false_type{});
                 waveSync();
             }
         }
@@ -288,7 +328,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     }
 
     waveSync();
-    if (valid) neighborsCount[i - firstId] = sh.count[w][lane];
+    if (valid) neighborsCount[i - firstId] = sh.target[w][lane].count;
 }
 
 template<class T>
@@ -308,16 +348,17 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
     size_t groups   = (size_t(lastId - firstId) + kWave - 1) / kWave;
     size_t threads  = groups * kWave;
     unsigned blocks = ceilDiv(threads, kNbThreads);
+    blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
     bool anyPbc     = box->bnd[0] == HIPCSTONE_PERIODIC || box->bnd[1] == HIPCSTONE_PERIODIC ||
                   box->bnd[2] == HIPCSTONE_PERIODIC;
     if (anyPbc)
         findNeighborsKernel<T, true><<<blocks, kNbThreads, 0, stream(s)>>>(
             x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
-            ngmax, neighbors, neighborsCount);
+            ngmax, neighbors, neighborsCount, blocks);
     else
         findNeighborsKernel<T, false><<<blocks, kNbThreads, 0, stream(s)>>>(
             x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
-            ngmax, neighbors, neighborsCount);
+            ngmax, neighbors, neighborsCount, blocks);
     HC_CHECK_LAUNCH("findNeighborsKernel");
     return HIPCSTONE_OK;
 }
