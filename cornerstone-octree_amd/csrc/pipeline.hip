@@ -1,6 +1,8 @@
 // Host orchestration of the plain single-domain pipeline (C++ above the kernels):
 //   keys -> stable sort + gather x,y,z,h -> updateOctree (computeOctree on the first call) -> link ->
-//   geometric centers -> leaf layout -> findNeighbors.
+//   geometric centers -> leaf layout -> findNeighbors,
+// and of one rank's share of the synced tree: makeSfcAssignment -> discoverHalos -> computeNodeLayout -> halo
+// gather (domain/domaindecomp.hpp:103-118, focus/octree_focus_mpi.hpp:511-582, halos/gather_halos_gpu.cu:28-57).
 //
 // This is the sequence of the reference's CPU pipeline (test/unit/neighbors/findneighbors.cpp:43-93,
 // test/performance/octree.cpp:29-100) and of one Domain::sync at numRanks=1 restricted to the global tree
@@ -147,6 +149,8 @@ struct PipelineBase
     virtual ~PipelineBase() = default;
     virtual hipcstone_status sync(hipcstone_stream s, const void* x, const void* y, const void* z, const void* h,
                                   size_t n, const void* box, int fullBuild, uint32_t* nb, uint32_t* nc) = 0;
+    virtual hipcstone_status halos(hipcstone_stream s, int32_t numRanks, int32_t rank, float ext, void* xo, void* yo,
+                                   void* zo, void* ho, size_t capacity, size_t* numLocal) = 0;
     virtual void view(hipcstone_pipeline_view* v) const = 0;
 };
 
@@ -189,6 +193,26 @@ struct Api;
         static hipcstone_status centers(hipcstone_stream s, const K* pf, int32_t N, T* c, T* sz, const Box* b)        \
         {                                                                                                              \
             return hipcstone_compute_geo_centers_##KS##_##RS(s, pf, N, c, sz, b);                                     \
+        }                                                                                                              \
+        static hipcstone_status assign(hipcstone_stream s, const K* leaves, const uint32_t* counts, int32_t L,        \
+                                       int32_t R, int32_t* off, K* bnd, uint32_t* rc, void* ws, size_t wsb)            \
+        {                                                                                                              \
+            return hipcstone_make_sfc_assignment_##KS(s, leaves, counts, L, R, off, bnd, rc, ws, wsb);                 \
+        }                                                                                                              \
+        static hipcstone_status discover(hipcstone_stream s, const K* pf, const int32_t* co, const int32_t* par,      \
+                                         const int32_t* l2i, const T* c, const T* sz, const K* leaves, int32_t N,      \
+                                         const T* x, const T* y, const T* z, const T* h, const uint32_t* lay,          \
+                                         int32_t first, int32_t last, const Box* b, float ext, T* sc, T* ss,           \
+                                         uint8_t* flags)                                                               \
+        {                                                                                                              \
+            return hipcstone_discover_halos_##KS##_##RS(s, pf, co, par, l2i, c, sz, leaves, N, x, y, z, h, lay, first, \
+                                                        last, b, ext, sc, ss, flags, 0);                               \
+        }                                                                                                              \
+        static hipcstone_status gatherLeaves(hipcstone_stream s, const uint32_t* src, const uint32_t* dst, int32_t L, \
+                                             const T* x, const T* y, const T* z, const T* h, T* xo, T* yo, T* zo,      \
+                                             T* ho)                                                                    \
+        {                                                                                                              \
+            return hipcstone_gather_leaf_ranges_##RS(s, src, dst, L, x, y, z, h, xo, yo, zo, ho);                     \
         }                                                                                                              \
         static hipcstone_status neighbors(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h,         \
                                           uint32_t f, uint32_t l, const Box* b, const int32_t* co,                    \
@@ -274,6 +298,70 @@ public:
         return HIPCSTONE_OK;
     }
 
+    hipcstone_status halos(hipcstone_stream s, int32_t numRanks, int32_t rank, float ext, void* xo, void* yo,
+                           void* zo, void* ho, size_t capacity, size_t* numLocal) override
+    {
+        if (numLeaves_ == 0 || numNodes_ == 0) return invalid("pipeline halos: no synced tree");
+        if (numRanks <= 0 || rank < 0 || rank >= numRanks) return invalid("pipeline halos: bad rank");
+        if (!numLocal) return invalid("pipeline halos: null num_local");
+        const int32_t L = numLeaves_, N = numNodes_;
+        size_t R = size_t(numRanks);
+        if (auto e = assignOff_.reserve((R + 1) * sizeof(int32_t))) return e;
+        if (auto e = assignBnd_.reserve((R + 1) * sizeof(K))) return e;
+        if (auto e = assignCnt_.reserve(R * sizeof(uint32_t))) return e;
+        if (auto e = assignWs_.reserve(hipcstone_sfc_assignment_workspace_size(L))) return e;
+        if (auto e = flags_.reserve(size_t(N))) return e;
+        if (auto e = searchC_.reserve(size_t(L) * 3 * sizeof(T))) return e;
+        if (auto e = searchS_.reserve(size_t(L) * 3 * sizeof(T))) return e;
+        if (auto e = localLayout_.reserve((size_t(L) + 1) * sizeof(uint32_t))) return e;
+        const int32_t* leafToInternalLeaves = leafToInternal_.as<int32_t>() + (N - L); // tree/octree.hpp:252
+        const T* x = xyzh_[cur_][0].template as<T>();
+        const T* y = xyzh_[cur_][1].template as<T>();
+        const T* z = xyzh_[cur_][2].template as<T>();
+        const T* h = xyzh_[cur_][3].template as<T>();
+        StageTimer& tm = this->timer;
+#define HC_STAGE(id, call)                                                                                             \
+    do {                                                                                                               \
+        if (auto e_ = tm.begin(id, s)) return e_;                                                                     \
+        if (auto e_ = (call)) return e_;                                                                               \
+        if (auto e_ = tm.end(s)) return e_;                                                                            \
+    } while (0)
+        HC_STAGE(7, A::assign(s, tree_[treeCur_].template as<K>(), counts_[treeCur_].template as<uint32_t>(), L,
+                              numRanks, assignOff_.as<int32_t>(), assignBnd_.as<K>(), assignCnt_.as<uint32_t>(),
+                              assignWs_.ptr, assignWs_.bytes));
+        int32_t range[2];
+        HC_CHECK(hipMemcpyAsync(range, assignOff_.as<int32_t>() + rank, sizeof(range), hipMemcpyDeviceToHost,
+                                stream(s)));
+        HC_CHECK(hipStreamSynchronize(stream(s)));
+        first_ = range[0];
+        last_  = range[1];
+        HC_STAGE(8, A::discover(s, prefixes_.as<K>(), childOffsets_.as<int32_t>(), parents_.as<int32_t>(),
+                                leafToInternalLeaves, centers_.as<T>(), sizes_.as<T>(), tree_[treeCur_].template as<K>(),
+                                N, x, y, z, h, layout_.as<uint32_t>(), first_, last_, &box_, ext, searchC_.as<T>(),
+                                searchS_.as<T>(), flags_.as<uint8_t>()));
+        HC_STAGE(9, hipcstone_compute_node_layout(s, counts_[treeCur_].template as<uint32_t>(), flags_.as<uint8_t>(),
+                                                  leafToInternalLeaves, L, first_, last_, localLayout_.as<uint32_t>(),
+                                                  scanWs_.ptr, scanWs_.bytes));
+        uint32_t total = 0;
+        HC_CHECK(hipMemcpyAsync(&total, localLayout_.as<uint32_t>() + L, sizeof(total), hipMemcpyDeviceToHost,
+                                stream(s)));
+        HC_CHECK(hipStreamSynchronize(stream(s)));
+        numLocal_ = total;
+        *numLocal = total;
+        if (!xo && !yo && !zo && !ho) return HIPCSTONE_OK;
+        if (!xo || !yo || !zo || !ho) return invalid("pipeline halos: give all four outputs or none");
+        if (capacity < total)
+        {
+            setError("pipeline halos: output capacity below the local particle count");
+            return HIPCSTONE_CAPACITY;
+        }
+        HC_STAGE(10, A::gatherLeaves(s, layout_.as<uint32_t>(), localLayout_.as<uint32_t>(), L, x, y, z, h,
+                                     static_cast<T*>(xo), static_cast<T*>(yo), static_cast<T*>(zo),
+                                     static_cast<T*>(ho)));
+#undef HC_STAGE
+        return HIPCSTONE_OK;
+    }
+
     void view(hipcstone_pipeline_view* v) const override
     {
         v->num_particles    = n_;
@@ -300,6 +388,14 @@ public:
         for (int i = 0; i < 24; ++i)
             v->level_range_host[i] = i < int(maxLevel<K>() + 2) ? levelRangeHost_[i] : 0;
         v->converged_calls = convergedCalls_;
+        v->assigned_first  = first_;
+        v->assigned_last   = last_;
+        v->num_local       = numLocal_;
+        v->tree_offsets    = assignOff_.ptr;
+        v->halo_flags      = flags_.ptr;
+        v->search_centers  = searchC_.ptr;
+        v->search_sizes    = searchS_.ptr;
+        v->local_layout    = localLayout_.ptr;
     }
 
 private:
@@ -436,12 +532,16 @@ private:
     int treeCur_       = 0;
     int convergedCalls_ = 0;
     int32_t levelRangeHost_[32]{};
+    int32_t first_     = 0;
+    int32_t last_      = 0;
+    size_t numLocal_   = 0;
 
     DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_;
     DeviceBuf xyzh_[2][4];
     DeviceBuf tree_[2], counts_[2], updWs_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_;
+    DeviceBuf assignOff_, assignBnd_, assignCnt_, assignWs_, flags_, searchC_, searchS_, localLayout_;
 };
 
 } // namespace hipcstone
@@ -488,6 +588,14 @@ hipcstone_status hipcstone_pipeline_sync(hipcstone_pipeline* p, hipcstone_stream
 {
     if (!p) return invalid("pipeline_sync: null handle");
     return p->impl->sync(s, x, y, z, h, n, box, full_build, neighbors, neighbors_count);
+}
+
+hipcstone_status hipcstone_pipeline_halos(hipcstone_pipeline* p, hipcstone_stream s, int32_t num_ranks, int32_t rank,
+                                          float search_ext, void* x_out, void* y_out, void* z_out, void* h_out,
+                                          size_t capacity, size_t* num_local)
+{
+    if (!p) return invalid("pipeline_halos: null handle");
+    return p->impl->halos(s, num_ranks, rank, search_ext, x_out, y_out, z_out, h_out, capacity, num_local);
 }
 
 hipcstone_status hipcstone_pipeline_set_timing(hipcstone_pipeline* p, int enable)

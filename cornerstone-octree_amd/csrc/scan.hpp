@@ -1,5 +1,6 @@
-// Device-wide exclusive scan of 32-bit integers (reduce, scan partials, re-scan with offsets).
-// Used for the rebalance op scan (tree/csarray.hpp:380), the leaf layout and the link offsets.
+// Device-wide exclusive scan of integers (reduce, scan partials, re-scan with offsets), input type In summed in
+// type V. Used for the rebalance op scan (tree/csarray.hpp:380), the leaf layouts and the 64-bit count scan of the
+// SFC assignment.
 #pragma once
 
 #include "common.hpp"
@@ -18,8 +19,12 @@ constexpr int kMidThreads           = 1024;
 
 inline size_t numTiles(size_t n) { return (n + kTile - 1) / kTile; }
 
-// bytes of workspace for a scan of n elements (partials + total)
-inline size_t workspaceBytes(size_t n) { return ((numTiles(n) + 1) * sizeof(uint32_t) + 255) & ~size_t(255); }
+// bytes of workspace for a scan of n elements summed in V (partials + total)
+template<class V = uint32_t>
+inline size_t workspaceBytes(size_t n)
+{
+    return ((numTiles(n) + 1) * sizeof(V) + 255) & ~size_t(255);
+}
 
 // inclusive scan across one wave (64 lanes) with shuffles
 template<class V>
@@ -59,9 +64,9 @@ __device__ inline V blockExclusiveScan(V v, V* total)
     return waveOff + incl - v;
 }
 
-template<class V>
-__global__ __launch_bounds__(kThreads) void tileReduceKernel(const V* __restrict__ in, size_t n,
-                                                             uint32_t* __restrict__ partials)
+template<class In, class V>
+__global__ __launch_bounds__(kThreads) void tileReduceKernel(const In* __restrict__ in, size_t n,
+                                                             V* __restrict__ partials)
 {
     size_t base = size_t(blockIdx.x) * kTile;
     V sum       = 0;
@@ -69,28 +74,29 @@ __global__ __launch_bounds__(kThreads) void tileReduceKernel(const V* __restrict
     for (int k = 0; k < kItems; ++k)
     {
         size_t i = base + size_t(k) * kThreads + threadIdx.x;
-        if (i < n) sum += in[i];
+        if (i < n) sum += V(in[i]);
     }
     V total;
     blockExclusiveScan<kThreads>(sum, &total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = uint32_t(total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
 // exclusive scan of the tile partials in place; partials[numTiles] = grand total
-static __global__ __launch_bounds__(kMidThreads) void partialsScanKernel(uint32_t* partials, int numTiles)
+template<class V>
+__global__ __launch_bounds__(kMidThreads) void partialsScanKernel(V* partials, int numTiles)
 {
-    uint32_t vals[kMaxPartialsPerThread];
-    uint32_t local = 0;
-    int base       = threadIdx.x * kMaxPartialsPerThread;
+    V vals[kMaxPartialsPerThread];
+    V local  = 0;
+    int base = threadIdx.x * kMaxPartialsPerThread;
 #pragma unroll
     for (int k = 0; k < kMaxPartialsPerThread; ++k)
     {
         int i   = base + k;
-        vals[k] = i < numTiles ? partials[i] : 0u;
+        vals[k] = i < numTiles ? partials[i] : V(0);
         local += vals[k];
     }
-    uint32_t total;
-    uint32_t run = blockExclusiveScan<kMidThreads>(local, &total);
+    V total;
+    V run = blockExclusiveScan<kMidThreads>(local, &total);
 #pragma unroll
     for (int k = 0; k < kMaxPartialsPerThread; ++k)
     {
@@ -103,9 +109,9 @@ static __global__ __launch_bounds__(kMidThreads) void partialsScanKernel(uint32_
 
 // out[i] = init + partials[tile] + exclusive prefix within the tile; out may alias in.
 // If writeTotalAt is non-null, *writeTotalAt = init + grand total.
-template<class V>
-__global__ __launch_bounds__(kThreads) void tileScanKernel(const V* in, V* out, size_t n, const uint32_t* partials,
-                                                           V init, V* writeTotalAt, int numTiles)
+template<class In, class V>
+__global__ __launch_bounds__(kThreads) void tileScanKernel(const In* in, V* out, size_t n, const V* partials, V init,
+                                                           V* writeTotalAt, int numTiles)
 {
     size_t base = size_t(blockIdx.x) * kTile + size_t(threadIdx.x) * kItems; // blocked for the local scan
     V vals[kItems];
@@ -114,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void tileScanKernel(const V* in, V* out, 
     for (int k = 0; k < kItems; ++k)
     {
         size_t i = base + k;
-        vals[k]  = i < n ? in[i] : V(0);
+        vals[k]  = i < n ? V(in[i]) : V(0);
         sum += vals[k];
     }
     V total;
@@ -129,26 +135,36 @@ __global__ __launch_bounds__(kThreads) void tileScanKernel(const V* in, V* out, 
     if (writeTotalAt && blockIdx.x == 0 && threadIdx.x == 0) *writeTotalAt = init + V(partials[numTiles]);
 }
 
-// Exclusive scan of n values of in into out (out may equal in). If totalOut is non-null it receives
-// init + sum(in), e.g. out[n] for a scan of n+1 slots. Up to 1024*16 tiles (67 M elements).
 template<class V>
-hipcstone_status exclusiveScan(hipStream_t s, const V* in, V* out, size_t n, V init, V* totalOut, void* ws,
+__global__ void storeValueKernel(V* dst, V v)
+{
+    *dst = v;
+}
+
+// Exclusive scan of n values of in (type In) into out (type V; out may equal in when the types match). If totalOut
+// is non-null it receives init + sum(in), e.g. out[n] for a scan of n+1 slots. Up to 1024*16 tiles (67 M elements).
+template<class V, class In = V>
+hipcstone_status exclusiveScan(hipStream_t s, const In* in, V* out, size_t n, V init, V* totalOut, void* ws,
                                size_t wsBytes)
 {
     if (n == 0)
     {
-        if (totalOut) HC_CHECK(hipMemcpyAsync(totalOut, &init, sizeof(V), hipMemcpyHostToDevice, s));
+        if (totalOut)
+        {
+            storeValueKernel<V><<<1, 1, 0, s>>>(totalOut, init);
+            HC_CHECK_LAUNCH("storeValueKernel");
+        }
         return HIPCSTONE_OK;
     }
     size_t tiles = numTiles(n);
     if (tiles > size_t(kMidThreads) * kMaxPartialsPerThread) return invalid("scan: too many elements");
-    if (!ws || wsBytes < workspaceBytes(n)) return invalid("scan: workspace too small");
-    uint32_t* partials = static_cast<uint32_t*>(ws);
-    tileReduceKernel<V><<<unsigned(tiles), kThreads, 0, s>>>(in, n, partials);
+    if (!ws || wsBytes < workspaceBytes<V>(n)) return invalid("scan: workspace too small");
+    V* partials = static_cast<V*>(ws);
+    tileReduceKernel<In, V><<<unsigned(tiles), kThreads, 0, s>>>(in, n, partials);
     HC_CHECK_LAUNCH("tileReduceKernel");
-    partialsScanKernel<<<1, kMidThreads, 0, s>>>(partials, int(tiles));
+    partialsScanKernel<V><<<1, kMidThreads, 0, s>>>(partials, int(tiles));
     HC_CHECK_LAUNCH("partialsScanKernel");
-    tileScanKernel<V><<<unsigned(tiles), kThreads, 0, s>>>(in, out, n, partials, init, totalOut, int(tiles));
+    tileScanKernel<In, V><<<unsigned(tiles), kThreads, 0, s>>>(in, out, n, partials, init, totalOut, int(tiles));
     HC_CHECK_LAUNCH("tileScanKernel");
     return HIPCSTONE_OK;
 }

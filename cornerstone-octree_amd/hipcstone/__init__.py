@@ -389,6 +389,87 @@ def find_halos(prefixes, child_offsets, parents, centers, sizes, leaves, search_
              C.c_int32(num_nodes), _p(flags)), "find_halos")
 
 
+def _dtype_of(a, default=None):
+    if isinstance(a, DeviceArray):
+        return np.dtype(a.dtype)
+    if default is None:
+        raise ValueError("raw device pointer: pass the element dtype")
+    return np.dtype(default)
+
+
+def make_sfc_assignment(leaves, counts, num_leaves, num_ranks, key_dtype=None, stream=None):
+    """makeSfcAssignment (domaindecomp.hpp:103-118): equal-count bins over the leaf array. Returns host arrays
+    (tree_offsets[num_ranks+1], boundaries[num_ranks+1], rank_counts[num_ranks])."""
+    K = _dtype_of(leaves, key_dtype)
+    _lib.hipcstone_sfc_assignment_workspace_size.restype = C.c_size_t
+    ws = DeviceArray(_lib.hipcstone_sfc_assignment_workspace_size(C.c_int32(num_leaves)), np.uint8)
+    off = DeviceArray(num_ranks + 1, np.int32)
+    bnd = DeviceArray(num_ranks + 1, K)
+    cnt = DeviceArray(num_ranks, np.uint32)
+    f = _fn(f"hipcstone_make_sfc_assignment_{_KS[K]}")
+    _check(f(_stream(stream), _p(leaves), _p(counts), C.c_int32(num_leaves), C.c_int32(num_ranks), _p(off), _p(bnd),
+             _p(cnt), _p(ws), C.c_size_t(ws.nbytes)), "make_sfc_assignment")
+    return off.download(), bnd.download(), cnt.download()
+
+
+def discover_halos(prefixes, child_offsets, parents, leaf_to_internal, centers, sizes, leaves, num_nodes, x, y, z, h,
+                   layout, first, last, box, search_centers, search_sizes, flags, search_ext=1.0, accumulate=False,
+                   key_dtype=None, stream=None):
+    """FocusedOctree::discoverHalos (octree_focus_mpi.hpp:511-568) on the device; leaf_to_internal is the leaf part
+    of the map and layout the absolute leaf layout of x..h."""
+    K = _dtype_of(prefixes, key_dtype)
+    T = np.dtype(box_dtype(box))
+    f = _fn(f"hipcstone_discover_halos_{_KS[K]}_{_RS[T]}")
+    _check(f(_stream(stream), _p(prefixes), _p(child_offsets), _p(parents), _p(leaf_to_internal), _p(centers),
+             _p(sizes), _p(leaves), C.c_int32(num_nodes), _p(x), _p(y), _p(z), _p(h), _p(layout), C.c_int32(first),
+             C.c_int32(last), C.byref(box), C.c_float(search_ext), _p(search_centers), _p(search_sizes), _p(flags),
+             C.c_int32(int(accumulate))), "discover_halos")
+
+
+def compute_node_layout(counts, flags, leaf_to_internal, num_leaves, first, last, layout, stream=None):
+    """computeNodeLayout (layout.hpp:153-185): layout[num_leaves+1] of the assigned + halo leaves."""
+    _lib.hipcstone_scan_workspace_size.restype = C.c_size_t
+    ws = DeviceArray(_lib.hipcstone_scan_workspace_size(C.c_size_t(num_leaves)), np.uint8)
+    _check(_lib.hipcstone_compute_node_layout(_stream(stream), _p(counts), _p(flags), _p(leaf_to_internal),
+                                              C.c_int32(num_leaves), C.c_int32(first), C.c_int32(last), _p(layout),
+                                              _p(ws), C.c_size_t(ws.nbytes)), "compute_node_layout")
+    return layout
+
+
+def extract_marked_elements(source, flags, first, second, source_dtype=None, stream=None):
+    """extractMarkedElements (layout.hpp:111-141): an (n, 2) host array of (source[a], source[b]) per marked run."""
+    V = _dtype_of(source, source_dtype)
+    _lib.hipcstone_extract_marked_workspace_size.restype = C.c_size_t
+    ws = DeviceArray(_lib.hipcstone_extract_marked_workspace_size(C.c_int32(second - first)), np.uint8)
+    cap = max(1, (second - first + 1) // 2 + 1)
+    pairs = DeviceArray(2 * cap, V)
+    num = DeviceArray(1, np.uint32)
+    f = _fn(f"hipcstone_extract_marked_elements_{_KS[V]}")
+    _check(f(_stream(stream), _p(source), _p(flags), C.c_int32(first), C.c_int32(second), _p(pairs),
+             C.c_uint32(cap), _p(num), _p(ws), C.c_size_t(ws.nbytes)), "extract_marked_elements")
+    n = int(num.download()[0])
+    return pairs.download()[: 2 * n].reshape(n, 2)
+
+
+def gather_ranges(range_scan, range_offsets, num_ranges, src, buffer, buffer_size, elem_bytes, stream=None):
+    """gatherRanges (gather_halos_gpu.h:24-31): buffer[t] = src[range_offsets[r] + t - range_scan[r]]."""
+    _check(_lib.hipcstone_gather_ranges(_stream(stream), _p(range_scan), _p(range_offsets), C.c_int32(num_ranges),
+                                        _p(src), _p(buffer), C.c_size_t(buffer_size), C.c_size_t(elem_bytes)),
+           "gather_ranges")
+
+
+def gather_leaf_ranges(src_layout, dst_layout, num_leaves, srcs, dsts, dtype=None, stream=None):
+    """Fused halo gather of up to four arrays from the full leaf layout to the local (kept-leaf) layout."""
+    srcs = list(srcs) + [None] * (4 - len(srcs))
+    dsts = list(dsts) + [None] * (4 - len(dsts))
+    T = _dtype_of(next(a for a in srcs if a is not None), dtype)
+    name = {np.dtype(np.float32): "f32", np.dtype(np.float64): "f64", np.dtype(np.uint32): "u32",
+            np.dtype(np.uint64): "u64"}[T]
+    f = _fn(f"hipcstone_gather_leaf_ranges_{name}")
+    _check(f(_stream(stream), _p(src_layout), _p(dst_layout), C.c_int32(num_leaves), *[_p(a) for a in srcs],
+             *[_p(a) for a in dsts]), "gather_leaf_ranges")
+
+
 def find_neighbors(x, y, z, h, first, last, box, child_offsets, parents, internal_to_leaf, layout, centers, sizes,
                    num_nodes, ngmax, neighbors, counts, ext=1.0, stream=None):
     """CPU-semantics radius search on the device (findneighbors.hpp:156-177)."""
@@ -411,6 +492,16 @@ def displace(x, y, z, h, n, scale, seed, step, box, stream=None):
                "synth_displace")
 
 
+def gaussian_mixture(x, y, z, h, n, offset=0, seed=7, clusters=64, background=0.2, target_neighbors=100.0,
+                     n_total=None, stream=None):
+    """Gaussian-mixture particles with density-matched h, generated on the device (hipcstone_synth_gaussian_mixture)."""
+    T = _dtype_of(x)
+    f = _fn(f"hipcstone_synth_gaussian_mixture_{_RS[T]}")
+    _check(f(_stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_size_t(n), C.c_uint64(offset), C.c_uint64(seed),
+             C.c_uint32(clusters), C.c_double(background), C.c_double(target_neighbors),
+             C.c_double(float(n if n_total is None else n_total))), "synth_gaussian_mixture")
+
+
 # ---------------------------------------------------------------- pipeline
 
 
@@ -424,7 +515,12 @@ class PipelineView(C.Structure):
     ] + [(name, C.c_void_p) for name in (
         "keys", "perm", "x", "y", "z", "h", "leaves", "counts", "prefixes", "child_offsets", "parents",
         "internal_to_leaf", "leaf_to_internal", "level_range", "centers", "sizes", "layout")] + [
-        ("level_range_host", C.c_int32 * 24)]
+        ("level_range_host", C.c_int32 * 24),
+        ("assigned_first", C.c_int32),
+        ("assigned_last", C.c_int32),
+        ("num_local", C.c_size_t),
+    ] + [(name, C.c_void_p) for name in (
+        "tree_offsets", "halo_flags", "search_centers", "search_sizes", "local_layout")]
 
 
 class Pipeline:
@@ -446,15 +542,26 @@ class Pipeline:
                                             C.byref(box), C.c_int(int(full_build)), _p(neighbors), _p(counts)),
                "pipeline_sync")
 
-    STAGES = ("keys", "sort_gather", "tree_update", "link", "centers", "layout", "neighbors")
+    STAGES = ("keys", "sort_gather", "tree_update", "link", "centers", "layout", "neighbors", "assignment",
+              "halo_discovery", "node_layout", "halo_gather")
+
+    def halos(self, num_ranks, rank, outputs=None, capacity=0, search_ext=1.0, stream=None):
+        """One rank's share of the synced tree: assignment, discoverHalos, node layout and (with outputs = four
+        device arrays) the halo gather. Returns the local particle count."""
+        nl = C.c_size_t()
+        outs = list(outputs) if outputs is not None else [None] * 4
+        _check(_lib.hipcstone_pipeline_halos(self.handle, _stream(stream), C.c_int32(num_ranks), C.c_int32(rank),
+                                             C.c_float(search_ext), *[_p(a) for a in outs], C.c_size_t(capacity),
+                                             C.byref(nl)), "pipeline_halos")
+        return nl.value
 
     def set_timing(self, enable=True):
         _check(_lib.hipcstone_pipeline_set_timing(self.handle, C.c_int(int(enable))), "pipeline_set_timing")
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since timing was enabled (synchronizes)."""
-        ms = (C.c_double * 7)()
-        cnt = (C.c_int32 * 7)()
+        ms = (C.c_double * len(self.STAGES))()
+        cnt = (C.c_int32 * len(self.STAGES))()
         _check(_lib.hipcstone_pipeline_stage_times(self.handle, ms, cnt), "pipeline_stage_times")
         return {name: (ms[i], cnt[i]) for i, name in enumerate(self.STAGES)}
 

@@ -263,6 +263,88 @@ HIPCSTONE_DECL_HALOS(u32, uint32_t, f64, double)
 HIPCSTONE_DECL_HALOS(u64, uint64_t, f32, float)
 HIPCSTONE_DECL_HALOS(u64, uint64_t, f64, double)
 
+/* ---- SFC assignment. Replaces makeSfcAssignment(numRanks, counts, tree), domain/domaindecomp.hpp:103-118, with
+ * uniformBins :32-55: bins of equal particle count over the leaf array, from a uint64 scan of the leaf counts.
+ * Outputs are device arrays: tree_offsets[num_ranks+1] (first leaf of each rank, SfcAssignment::treeOffsets),
+ * boundaries[num_ranks+1] = leaves[tree_offsets] (the rank boundary keys) and rank_counts[num_ranks] (particles per
+ * rank, uint32 as in the reference). Workspace: hipcstone_sfc_assignment_workspace_size(num_leaves). ---- */
+size_t hipcstone_sfc_assignment_workspace_size(int32_t num_leaves);
+#define HIPCSTONE_DECL_ASSIGN(KS, K)                                                                                   \
+    hipcstone_status hipcstone_make_sfc_assignment_##KS(hipcstone_stream s, const K* leaves, const uint32_t* counts,  \
+                                                        int32_t num_leaves, int32_t num_ranks, int32_t* tree_offsets, \
+                                                        K* boundaries, uint32_t* rank_counts, void* workspace,         \
+                                                        size_t workspace_bytes);
+HIPCSTONE_DECL_ASSIGN(u32, uint32_t)
+HIPCSTONE_DECL_ASSIGN(u64, uint64_t)
+
+/* ---- Halo discovery of one rank. Replaces FocusedOctree::discoverHalos(x, y, z, h, layout, searchExtFact, scratch,
+ * accumulate), focus/octree_focus_mpi.hpp:511-568: search boxes of the leaves [first_node, last_node) are seeded with
+ * the leaves' geometric centers (node_centers gathered by leaf_to_internal, the leaf part of the map:
+ * leafToInternal + numInternalNodes, tree/octree.hpp:252), grown to the AABB of the spheres of
+ * radius 2*search_ext*h of their particles (computeBoundingBox), flags are zeroed unless accumulate, and findHalos
+ * marks every node that overlaps a search box and lies outside [leaves[first_node], leaves[last_node]).
+ * Difference: layout holds the absolute offsets of every leaf's particles in x..h (the reference scans the counts of
+ * the assigned leaves into a layout relative to the first assigned particle). search_centers / search_sizes are
+ * leaf-indexed Vec3 arrays; only [first_node, last_node) is written. ---- */
+#define HIPCSTONE_DECL_DISCOVER(KS, K, RS, T)                                                                          \
+    hipcstone_status hipcstone_discover_halos_##KS##_##RS(                                                             \
+        hipcstone_stream s, const K* prefixes, const int32_t* child_offsets, const int32_t* parents,                  \
+        const int32_t* leaf_to_internal, const T* node_centers, const T* node_sizes, const K* leaves,                  \
+        int32_t num_nodes, const T* x, const T* y, const T* z, const T* h, const uint32_t* layout,                     \
+        int32_t first_node, int32_t last_node, const hipcstone_box_##RS* box, float search_ext, T* search_centers,     \
+        T* search_sizes, uint8_t* flags, int32_t accumulate);
+HIPCSTONE_DECL_DISCOVER(u32, uint32_t, f32, float)
+HIPCSTONE_DECL_DISCOVER(u32, uint32_t, f64, double)
+HIPCSTONE_DECL_DISCOVER(u64, uint64_t, f32, float)
+HIPCSTONE_DECL_DISCOVER(u64, uint64_t, f64, double)
+
+/* ---- a15: local particle layout. Replaces computeNodeLayout(exec, focusLeafCounts, flags, leafToInternal, idx,
+ * layout), domain/layout.hpp:153-185: leaf i keeps counts[i] particles iff first <= i < last or
+ * flags[leaf_to_internal[i]] != 0, else 0 (leaf_to_internal is the leaf part of the map,
+ * tree/octree.hpp:372-375); layout[num_leaves+1] is the exclusive scan of the kept counts, the total in
+ * layout[num_leaves]. Workspace: hipcstone_scan_workspace_size(num_leaves). ---- */
+hipcstone_status hipcstone_compute_node_layout(hipcstone_stream s, const uint32_t* counts, const uint8_t* flags,
+                                               const int32_t* leaf_to_internal, int32_t num_leaves, int32_t first,
+                                               int32_t last, uint32_t* layout, void* workspace,
+                                               size_t workspace_bytes);
+
+/* ---- a15: marked ranges. Replaces extractMarkedElements(source, flags, firstReqIdx, secondReqIdx),
+ * domain/layout.hpp:111-141 (a host std::vector there): every maximal run [a, b) of indices in [first, second) with
+ * flags[i+1] > flags[i] gives the pair (source[a], source[b]), in order. flags must be nondecreasing (a layout).
+ * pairs receives min(num_pairs, max_pairs) pairs; *num_pairs (device) receives the number of runs. ---- */
+size_t hipcstone_extract_marked_workspace_size(int32_t num_elements);
+#define HIPCSTONE_DECL_EXTRACT(VS, V)                                                                                  \
+    hipcstone_status hipcstone_extract_marked_elements_##VS(hipcstone_stream s, const V* source,                      \
+                                                            const uint32_t* flags, int32_t first, int32_t second,     \
+                                                            V* pairs, uint32_t max_pairs, uint32_t* num_pairs,        \
+                                                            void* workspace, size_t workspace_bytes);
+HIPCSTONE_DECL_EXTRACT(u32, uint32_t)
+HIPCSTONE_DECL_EXTRACT(u64, uint64_t)
+
+/* ---- a15: range gather. Replaces gatherRanges(exec, rangeScan, rangeOffsets, numRanges, src, buffer, bufferSize),
+ * halos/gather_halos_gpu.h:24-31 (kernel gather_halos_gpu.cu:28-44): buffer[t] = src[range_offsets[r] + t -
+ * range_scan[r]] with r the last range whose range_scan[r] <= t. Elements are opaque records of elem_bytes in
+ * {1, 2, 4, 8, 12, 16} (the reference instantiates int and float arrays of 1..4). range_scan is nondecreasing with
+ * range_scan[0] = 0; buffer_size < 2^32. ---- */
+hipcstone_status hipcstone_gather_ranges(hipcstone_stream s, const uint32_t* range_scan,
+                                         const uint32_t* range_offsets, int32_t num_ranges, const void* src,
+                                         void* buffer, size_t buffer_size, size_t elem_bytes);
+
+/* ---- a15: fused halo gather (no reference counterpart as one call; it is extractMarkedElements + gatherRanges
+ * over the leaves kept by computeNodeLayout, for up to four arrays at once): for every leaf, the
+ * dst_layout[i+1] - dst_layout[i] particles starting at src_layout[i] are copied to dst_layout[i]. src_layout is the
+ * leaf layout of the full arrays, dst_layout the one from hipcstone_compute_node_layout. NULL source arrays are
+ * skipped. ---- */
+#define HIPCSTONE_DECL_GATHER_LEAVES(TS, T)                                                                            \
+    hipcstone_status hipcstone_gather_leaf_ranges_##TS(hipcstone_stream s, const uint32_t* src_layout,                \
+                                                       const uint32_t* dst_layout, int32_t num_leaves, const T* src0, \
+                                                       const T* src1, const T* src2, const T* src3, T* dst0, T* dst1, \
+                                                       T* dst2, T* dst3);
+HIPCSTONE_DECL_GATHER_LEAVES(f32, float)
+HIPCSTONE_DECL_GATHER_LEAVES(f64, double)
+HIPCSTONE_DECL_GATHER_LEAVES(u32, uint32_t)
+HIPCSTONE_DECL_GATHER_LEAVES(u64, uint64_t)
+
 /* ---- a16: radius neighbor search with the CPU semantics of findNeighbors(x, y, z, h, firstId, lastId, box,
  * OctreeNsView, ngmax, neighbors, neighborsCount), findneighbors.hpp:156-177 — the reference has no GPU free
  * function with this contract; its GPU search is GpuAlwaysTraverseNeighborhoodBuilder
@@ -314,6 +396,15 @@ typedef struct
     void* sizes;
     void* layout; /* first particle of each leaf, num_leaves + 1 (u32) */
     int32_t level_range_host[24];
+    /* state of the last hipcstone_pipeline_halos call (NULL / 0 before the first) */
+    int32_t assigned_first;   /* first leaf of the rank */
+    int32_t assigned_last;    /* one past its last leaf */
+    size_t num_local;         /* assigned + halo particles */
+    void* tree_offsets;       /* first leaf of every rank, num_ranks + 1 (i32) */
+    void* halo_flags;         /* node-indexed halo flags (u8) */
+    void* search_centers;     /* leaf-indexed search boxes of the assigned leaves, Vec3 (T) */
+    void* search_sizes;
+    void* local_layout;       /* first local particle of each leaf, num_leaves + 1 (u32) */
 } hipcstone_pipeline_view;
 
 hipcstone_status hipcstone_pipeline_create(hipcstone_pipeline** p, int curve, int key_bits, int real_bits,
@@ -324,11 +415,22 @@ hipcstone_status hipcstone_pipeline_sync(hipcstone_pipeline* p, hipcstone_stream
                                          uint32_t* neighbors, uint32_t* neighbors_count);
 hipcstone_status hipcstone_pipeline_get_view(const hipcstone_pipeline* p, hipcstone_pipeline_view* v);
 
+/* One rank's share of the synced tree: makeSfcAssignment(num_ranks) over the leaf counts, then for `rank`
+ * discoverHalos with search_ext (octree_focus_mpi.hpp:511-568), computeNodeLayout (layout.hpp:153-185) and the halo
+ * gather of x,y,z,h into x_out..h_out in SFC order (assigned + halo particles; the packing of gatherRanges,
+ * gather_halos_gpu.cu:28-57). *num_local receives the local particle count; with capacity below it (or NULL
+ * outputs) nothing is gathered and HIPCSTONE_CAPACITY (resp. OK) is returned. Synchronizes twice (rank range,
+ * local count). ---- */
+hipcstone_status hipcstone_pipeline_halos(hipcstone_pipeline* p, hipcstone_stream s, int32_t num_ranks, int32_t rank,
+                                          float search_ext, void* x_out, void* y_out, void* z_out, void* h_out,
+                                          size_t capacity, size_t* num_local);
+
 /* Per-stage timing of sync steps with HIP events recorded on the step's stream (no extra synchronization while
- * enabled). Stages: 0 keys, 1 sort+gather, 2 tree update, 3 link, 4 centers, 5 layout, 6 neighbors.
+ * enabled). Stages: 0 keys, 1 sort+gather, 2 tree update, 3 link, 4 centers, 5 layout, 6 neighbors, and of
+ * hipcstone_pipeline_halos: 7 assignment, 8 halo discovery, 9 node layout, 10 halo gather.
  * hipcstone_pipeline_stage_times synchronizes, then returns, per stage, the summed milliseconds and the number of
  * timed launches since timing was enabled (arrays of HIPCSTONE_PIPELINE_STAGES entries). */
-#define HIPCSTONE_PIPELINE_STAGES 7
+#define HIPCSTONE_PIPELINE_STAGES 11
 hipcstone_status hipcstone_pipeline_set_timing(hipcstone_pipeline* p, int enable);
 hipcstone_status hipcstone_pipeline_stage_times(hipcstone_pipeline* p, double* ms, int32_t* launches);
 
@@ -340,6 +442,18 @@ hipcstone_status hipcstone_synth_displace_f32(hipcstone_stream s, float* x, floa
 hipcstone_status hipcstone_synth_displace_f64(hipcstone_stream s, double* x, double* y, double* z, const double* h,
                                               size_t n, double scale, uint64_t seed, uint64_t step,
                                               const hipcstone_box_f64* box);
+
+/* ---- synthetic workload helper (not an octree stage): Gaussian-mixture particles in [0,1)^3 — `clusters`
+ * isotropic Gaussians (centres uniform, sigma log-uniform in [0.005, 0.05]) plus a uniform `background` fraction,
+ * clipped to [0,1) — with h = 0.5 (target_neighbors / (4/3 pi rho))^(1/3) from the density rho of each particle's
+ * own component for n_total particles. Particle i is element offset + i of counter-based splitmix64 streams, so a
+ * large set can be made in slices. ---- */
+hipcstone_status hipcstone_synth_gaussian_mixture_f32(hipcstone_stream s, float* x, float* y, float* z, float* h,
+                                                      size_t n, uint64_t offset, uint64_t seed, uint32_t clusters,
+                                                      double background, double target_neighbors, double n_total);
+hipcstone_status hipcstone_synth_gaussian_mixture_f64(hipcstone_stream s, double* x, double* y, double* z, double* h,
+                                                      size_t n, uint64_t offset, uint64_t seed, uint32_t clusters,
+                                                      double background, double target_neighbors, double n_total);
 
 #ifdef __cplusplus
 }

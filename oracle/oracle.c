@@ -22,6 +22,10 @@
  *   neighbors brute force all-pairs (all_to_all.hpp:26-65) on small inputs, _ref findNeighbors
  *   halos     brute force all-to-all (discovery.cpp:25-50), _ref findHalos
  *   bbox      direct min/max, _ref computeBoundingBox
+ *   assign    _ref makeSfcAssignment, equal-count bounds
+ *   layout    _ref computeNodeLayout (CPU branch), brute-force kept counts
+ *   extract   _ref extractMarkedElements, brute-force run enumeration
+ *   gather    brute-force element copy and range enumeration (the reference gatherRanges is GPU-only, not built)
  */
 
 #include <math.h>
@@ -169,6 +173,56 @@ void orc_gather_u64(const uint32_t* perm, size_t n, const uint64_t* src, uint64_
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i)
         dst[i] = src[perm[i]];
+}
+
+/*
+ * domain/layout.hpp:153-185 computeNodeLayout (CPU branch :174-182): leaf i keeps its count iff
+ * first <= i < last or flags[leafToInternal[i]]; layout = exclusive scan of the kept counts, size numLeaves + 1.
+ */
+void orc_compute_node_layout(const uint32_t* counts, const uint8_t* flags, const int32_t* leafToInternal,
+                             int numLeaves, int first, int last, uint32_t* layout)
+{
+    uint32_t acc = 0;
+    for (int i = 0; i < numLeaves; ++i)
+    {
+        int have  = (first <= i && i < last) || flags[leafToInternal[i]];
+        layout[i] = acc;
+        acc += have ? counts[i] : 0u;
+    }
+    layout[numLeaves] = acc;
+}
+
+/*
+ * halos/gather_halos_gpu.cu:28-44 gatherRangesKernel, as a loop over ranges: range r covers buffer positions
+ * [rangeScan[r], rangeScan[r+1]) (the last one up to bufferSize) and copies them from src starting at
+ * rangeOffsets[r]. Elements are elemBytes-byte records.
+ */
+void orc_gather_ranges(const uint32_t* rangeScan, const uint32_t* rangeOffsets, int numRanges, const void* src,
+                       void* buffer, size_t bufferSize, size_t elemBytes)
+{
+    const unsigned char* s = (const unsigned char*)src;
+    unsigned char* b       = (unsigned char*)buffer;
+    for (int r = 0; r < numRanges; ++r)
+    {
+        size_t lo = rangeScan[r];
+        size_t hi = r + 1 < numRanges ? rangeScan[r + 1] : bufferSize;
+        if (hi > bufferSize) hi = bufferSize;
+        for (size_t t = lo; t < hi; ++t)
+            memcpy(b + t * elemBytes, s + ((size_t)rangeOffsets[r] + (t - lo)) * elemBytes, elemBytes);
+    }
+}
+
+/* gather the particles of every kept leaf from the full arrays (srcLayout) to the local arrays (dstLayout) */
+void orc_gather_leaf_ranges(const uint32_t* srcLayout, const uint32_t* dstLayout, int numLeaves, const void* src,
+                            void* dst, size_t elemBytes)
+{
+    const unsigned char* s = (const unsigned char*)src;
+    unsigned char* d       = (unsigned char*)dst;
+    for (int i = 0; i < numLeaves; ++i)
+    {
+        size_t n = dstLayout[i + 1] - dstLayout[i];
+        memcpy(d + (size_t)dstLayout[i] * elemBytes, s + (size_t)srcLayout[i] * elemBytes, n * elemBytes);
+    }
 }
 
 void orc_free(void* p) { free(p); }

@@ -255,6 +255,97 @@ class CpuLib:
           _ptr(times))
         return t[: nn.value + 1].copy(), c[: nn.value].copy(), times, nc[:m]
 
+    # ---- domain decomposition and halo layout
+    def make_sfc_assignment(self, tree, counts, num_ranks):
+        """makeSfcAssignment (domaindecomp.hpp:103-118): (treeOffsets, boundaries, rank counts)."""
+        tree = np.ascontiguousarray(tree)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        L = counts.size
+        off = np.zeros(num_ranks + 1, dtype=np.int32)
+        bnd = np.zeros(num_ranks + 1, dtype=tree.dtype)
+        rc = np.zeros(num_ranks, dtype=np.uint32)
+        self._f(f"make_sfc_assignment_{_KS[tree.dtype]}")(_ptr(tree), _ptr(counts), C.c_int(L), C.c_int(num_ranks),
+                                                          _ptr(off), _ptr(bnd), _ptr(rc))
+        return off, bnd, rc
+
+    def extract_marked_elements(self, source, flags, first, second):
+        """extractMarkedElements (layout.hpp:111-141) as an (n, 2) array of pairs."""
+        source = np.ascontiguousarray(source)
+        flags = np.ascontiguousarray(flags, dtype=np.uint32)
+        cap = max(1, (second - first + 1) // 2 + 1)
+        pairs = np.zeros(2 * cap, dtype=source.dtype)
+        f = self._f(f"extract_marked_elements_{_KS[source.dtype]}")
+        if self.kind == "oracle":
+            f.restype = C.c_size_t
+            n = f(_ptr(source), _ptr(flags), C.c_int(first), C.c_int(second), _ptr(pairs), C.c_size_t(cap))
+        else:
+            f.restype = C.c_size_t
+            n = f(_ptr(source), C.c_size_t(source.size), _ptr(flags), C.c_size_t(flags.size), C.c_int(first),
+                  C.c_int(second), _ptr(pairs), C.c_size_t(cap))
+        assert n <= cap
+        return pairs[: 2 * n].reshape(n, 2).copy()
+
+    def compute_node_layout(self, counts, flags, leaf_to_internal, first, last):
+        """computeNodeLayout (layout.hpp:153-185), CPU branch: layout of numLeaves + 1 entries. leaf_to_internal is
+        the leaf part of the map (leafToInternal + numInternalNodes, octree.hpp:372-375)."""
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        flags = np.ascontiguousarray(flags, dtype=np.uint8)
+        l2i = np.ascontiguousarray(leaf_to_internal, dtype=np.int32)
+        L = counts.size
+        layout = np.zeros(L + 1, dtype=np.uint32)
+        if self.kind == "oracle":
+            self.lib.orc_compute_node_layout(_ptr(counts), _ptr(flags), _ptr(l2i), C.c_int(L), C.c_int(first),
+                                             C.c_int(last), _ptr(layout))
+        else:
+            self.lib.ref_compute_node_layout(_ptr(counts), _ptr(flags), _ptr(l2i), C.c_int(L), C.c_int(flags.size),
+                                             C.c_int(first), C.c_int(last), _ptr(layout))
+        return layout
+
+    def gather_ranges(self, range_scan, range_offsets, src, buffer_size):
+        """gatherRanges (gather_halos_gpu.cu:28-44) restated as a loop over ranges (oracle only)."""
+        assert self.kind == "oracle"
+        rs = np.ascontiguousarray(range_scan, dtype=np.uint32)
+        ro = np.ascontiguousarray(range_offsets, dtype=np.uint32)
+        src = np.ascontiguousarray(src)
+        out = np.zeros((buffer_size,) + src.shape[1:], dtype=src.dtype)
+        elem = src.dtype.itemsize * int(np.prod(src.shape[1:], dtype=np.int64))
+        self.lib.orc_gather_ranges(_ptr(rs), _ptr(ro), C.c_int(rs.size), _ptr(src), _ptr(out),
+                                   C.c_size_t(buffer_size), C.c_size_t(elem))
+        return out
+
+    def gather_leaf_ranges(self, src_layout, dst_layout, src):
+        """Kept-leaf gather from the full arrays to the local layout (oracle only)."""
+        assert self.kind == "oracle"
+        sl = np.ascontiguousarray(src_layout, dtype=np.uint32)
+        dl = np.ascontiguousarray(dst_layout, dtype=np.uint32)
+        src = np.ascontiguousarray(src)
+        out = np.zeros(int(dl[-1]), dtype=src.dtype)
+        self.lib.orc_gather_leaf_ranges(_ptr(sl), _ptr(dl), C.c_int(dl.size - 1), _ptr(src), _ptr(out),
+                                        C.c_size_t(src.dtype.itemsize))
+        return out
+
+    def discover_halos(self, tree, centers, sizes, leaves, x, y, z, h, layout, first, last, box, search_ext=1.0,
+                       accumulate_flags=None):
+        """discoverHalos (octree_focus_mpi.hpp:511-568) over the leaf layout (oracle only):
+        (searchCenters, searchSizes, flags)."""
+        assert self.kind == "oracle"
+        T = x.dtype
+        K = leaves.dtype
+        N = tree["prefixes"].size
+        L = leaves.size - 1
+        sc = np.zeros((L, 3), dtype=T)
+        ss = np.zeros((L, 3), dtype=T)
+        flags = (np.zeros(N, dtype=np.uint8) if accumulate_flags is None
+                 else np.array(accumulate_flags, dtype=np.uint8, copy=True))
+        l2i = np.ascontiguousarray(tree["leafToInternal"][N - L:])  # the leaf part, octree.hpp:252
+        self._f(f"discover_halos_{_KS[K]}_{_RS[T]}")(
+            _ptr(tree["prefixes"]), _ptr(tree["childOffsets"]), _ptr(tree["parents"]), _ptr(l2i),
+            _ptr(np.ascontiguousarray(centers)), _ptr(np.ascontiguousarray(sizes)), _ptr(leaves), C.c_int(N),
+            _ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(np.ascontiguousarray(layout, dtype=np.uint32)),
+            C.c_int(first), C.c_int(last), C.byref(box), C.c_float(search_ext), _ptr(sc), _ptr(ss), _ptr(flags),
+            C.c_int(0 if accumulate_flags is None else 1))
+        return sc, ss, flags
+
     def omp_threads(self):
         assert self.kind == "ref"
         return int(self.lib.ref_omp_threads())

@@ -13,6 +13,8 @@
 #include <utility>
 #include <vector>
 
+#include "cstone/domain/domaindecomp.hpp"
+#include "cstone/domain/layout.hpp"
 #include "cstone/findneighbors.hpp"
 #include "cstone/focus/source_center.hpp"
 #include "cstone/primitives/gather.hpp"
@@ -295,6 +297,43 @@ extern "C"
 
     REF_KEYS(uint32_t, u32)
     REF_KEYS(uint64_t, u64)
+
+#define REF_DOMAIN(K, KS)                                                                                              \
+    void ref_make_sfc_assignment_##KS(const K* tree, const unsigned* counts, int numLeaves, int numRanks,             \
+                                      int32_t* treeOffsets, K* boundaries, uint32_t* rankCounts)                       \
+    {                                                                                                                  \
+        std::vector<unsigned> c(counts, counts + numLeaves);                                                           \
+        auto a = makeSfcAssignment(numRanks, c, tree);                                                                 \
+        for (int r = 0; r <= numRanks; ++r)                                                                            \
+        {                                                                                                              \
+            treeOffsets[r] = a.treeOffsetsConst()[r];                                                                  \
+            boundaries[r]  = a[r];                                                                                     \
+        }                                                                                                              \
+        for (int r = 0; r < numRanks; ++r)                                                                             \
+            rankCounts[r] = a.totalCount(r);                                                                           \
+    }                                                                                                                  \
+    size_t ref_extract_marked_elements_##KS(const K* source, size_t sourceSize, const uint32_t* flags,                \
+                                            size_t flagsSize, int32_t first, int32_t second, K* pairs,                 \
+                                            size_t maxPairs)                                                           \
+    {                                                                                                                  \
+        auto v = extractMarkedElements<K>(std::span<const K>(source, sourceSize),                                      \
+                                          std::span<const LocalIndex>(flags, flagsSize), first, second);               \
+        for (size_t k = 0; k < v.size() && k < 2 * maxPairs; ++k)                                                      \
+            pairs[k] = v[k];                                                                                           \
+        return v.size() / 2;                                                                                           \
+    }
+
+    REF_DOMAIN(uint32_t, u32)
+    REF_DOMAIN(uint64_t, u64)
+
+    void ref_compute_node_layout(const unsigned* counts, const uint8_t* flags, const int32_t* leafToInternal,
+                                 int numLeaves, int numNodes, int first, int last, uint32_t* layout)
+    {
+        computeNodeLayout(execution::Cpu{}, std::span<const unsigned>(counts, numLeaves),
+                          std::span<const uint8_t>(flags, numNodes),
+                          std::span<const TreeNodeIndex>(leafToInternal, numLeaves), TreeIndexPair(first, last),
+                          std::span<LocalIndex>(layout, size_t(numLeaves) + 1));
+    }
 
 #define REF_KR(K, KS, T, TS)                                                                                           \
     void ref_node_fp_centers_##KS##_##TS(int curve, const K* prefixes, size_t n, T* centers, T* sizes,               \
