@@ -91,6 +91,79 @@ def load_traffic():
     return None
 
 
+def halo_bench(args):
+    """Rank-share workload: 134,217,728 fp64 Gaussian-mixture particles (64 clusters, 20 % background, h for ~100
+    neighbors from the local component density), Morton-64 keys, bucketSize 128; one step = keys + sort/gather +
+    computeOctree from the root + link + centers + layout + makeSfcAssignment(8) + discoverHalos of rank 2
+    (search boxes 2h) + node layout + halo gather of x,y,z,h. Inputs are generated on the device and stay resident."""
+    import hipcstone as hip
+
+    n, K, W = args.n_halo, args.steps, args.warmup
+    ranks, rank = 8, 2
+    T = np.float64
+    d = [hip.DeviceArray(n, T) for _ in range(4)]
+    hip.gaussian_mixture(*d, n, seed=7, clusters=64, background=0.2, target_neighbors=100.0)
+    box = hip.make_box(0.0, 1.0, T)
+    pipe = hip.Pipeline(curve=hip.MORTON, key_bits=64, real_bits=64, bucket_size=128, ngmax=0)
+    pipe.sync(*d, n, box, full_build=True)
+    m = pipe.halos(ranks, rank)
+    cap = int(m * 1.05) + 1024
+    outs = [hip.DeviceArray(cap, T) for _ in range(4)]
+    log(f"input: {n} mixture particles; local share of rank {rank}: {m} particles")
+
+    def step():
+        pipe.sync(*d, n, box, full_build=True)
+        return pipe.halos(ranks, rank, outs, cap)
+
+    for _ in range(W):
+        step()
+    hip.device_sync()
+    events = [hip.Event() for _ in range(K + 1)]
+    pipe.set_timing(True)
+    hip.device_sync()
+    events[0].record()
+    locals_ = []
+    for k in range(K):
+        locals_.append(step())
+        events[k + 1].record()
+    hip.device_sync()
+    per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
+    total_ms = events[0].elapsed_ms(events[K])
+    stages = pipe.stage_times()
+    pipe.set_timing(False)
+    v = pipe.view()
+    assert len(set(locals_)) == 1, "rank share changed between identical steps"
+    ms_per_step = total_ms / K
+    assigned = int(hip.download_ptr(v.layout, v.num_leaves + 1, np.uint32)[v.assigned_last]) - int(
+        hip.download_ptr(v.layout, v.num_leaves + 1, np.uint32)[v.assigned_first])
+    # compulsory bytes per step: keys 24 read + 8 write, sort 8 read + 12 write, gather 4 + 32 + 32, leaf array and
+    # link/centers ~2 per particle, search boxes over the assigned particles 32, halo gather 64 per local particle
+    step_bytes = n * (32 + 20 + 68 + 2) + 32 * assigned + 64 * m
+    result = {
+        "metric": "Mparticles/s rank-share chain keys->sort->tree->assignment->findHalos->halo gather (134M fp64)",
+        "value": round(n * K / (total_ms / 1e3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": (f"rank {rank} of {ranks} share at real size: {n:,} fp64 Gaussian-mixture particles "
+                         "(64 clusters, sigma log-uniform [0.005,0.05], 20% background, [0,1)^3), Morton-64 keys, "
+                         "bucketSize 128, full build each step, discoverHalos with 2h search boxes, halo gather"),
+            "particles": n, "bucket_size": 128, "keys": "morton-64", "ranks": ranks, "rank": rank,
+            "leaves": int(v.num_leaves), "nodes": int(v.num_nodes), "assigned_leaves": v.assigned_last - v.assigned_first,
+            "assigned_particles": assigned, "local_particles": int(m),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
+                       "p99": round(float(np.percentile(per_step, 99)), 3), "max": round(float(per_step.max()), 3)},
+        "pipeline_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                              "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
+                              "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()
+                   if c > 0},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +173,9 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["plain", "halo"], default="plain",
+                    help="plain: the 16.8M neighbor pipeline (default); halo: the 134M rank-share chain")
+    ap.add_argument("--n-halo", type=int, default=134217728)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (reference reading for HBM traffic counters)")
     args = ap.parse_args()
@@ -125,6 +201,10 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "of the pipeline is a design in DESIGN.md, not a runnable path", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "halo":
+        halo_bench(args)
+        return
 
     import hipcstone as hip
     from hipcstone import inputs

@@ -341,6 +341,32 @@ __device__ inline double readLane(double v, int lane)
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ inline unsigned uniform(unsigned v) { return unsigned(__builtin_amdgcn_readfirstlane(int(v))); }
 
+__device__ inline uint64_t uniform64(uint64_t v)
+{
+    uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
+    uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// order LDS writes of some lanes before LDS reads of other lanes of the same wave
+__device__ inline void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// XCD-aware work order: workgroups are dealt round-robin to the 8 XCDs (each with its own L2); remap so that each
+// XCD walks one contiguous range of groups, whose neighbouring work items share tree nodes in its L2.
+// numBlocksPadded is a multiple of kXcds.
+constexpr uint32_t kXcds = 8;
+
+__device__ inline uint32_t xcdContiguousBlock(uint32_t b, uint32_t numBlocksPadded)
+{
+    uint32_t perXcd = numBlocksPadded / kXcds;
+    return (b % kXcds) * perXcd + b / kXcds;
+}
+
 inline unsigned ceilDiv(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
 
 } // namespace hipcstone

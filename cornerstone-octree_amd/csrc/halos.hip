@@ -96,81 +96,167 @@ __device__ inline bool boxContainedInRange(K codeStart, K codeEnd, const T c[3],
     return es >= codeStart && ee <= codeEnd;
 }
 
-// findHalos: one lane per local leaf, each with its own stackless walk (collisions.hpp:25-48). Flags only ever
-// receive 1, so concurrent writers agree. The walk is capped at 2 * numNodes + 8 steps.
-template<class K, class T>
-__global__ __launch_bounds__(128) void findHalosKernel(const K* __restrict__ prefixes,
-                                                       const int32_t* __restrict__ childOffsets,
-                                                       const int32_t* __restrict__ parents,
-                                                       const T* __restrict__ nodeCenters,
-                                                       const T* __restrict__ nodeSizes, const K* __restrict__ leaves,
-                                                       const T* __restrict__ searchCenters,
-                                                       const T* __restrict__ searchSizes,
-                                                       typename BoxOf<T>::type box, int32_t firstNode,
-                                                       int32_t lastNode, int32_t numNodes, uint8_t* flags)
-{
-    int32_t leafIdx = firstNode + int32_t(size_t(blockIdx.x) * blockDim.x + threadIdx.x);
-    if (leafIdx >= lastNode) return;
-    K lowestKey  = leaves[firstNode];
-    K highestKey = leaves[lastNode];
-    T tC[3] = {searchCenters[size_t(leafIdx) * 3], searchCenters[size_t(leafIdx) * 3 + 1],
-               searchCenters[size_t(leafIdx) * 3 + 2]};
-    T tS[3] = {searchSizes[size_t(leafIdx) * 3], searchSizes[size_t(leafIdx) * 3 + 1],
-               searchSizes[size_t(leafIdx) * 3 + 2]};
-    if (boxContainedInRange<K, T>(lowestKey, highestKey, tC, tS, box)) return;
+// findHalos (collisions.hpp:25-94): for each local leaf, a walk from the root that calls, at every node it reaches,
+// overlaps(node) = !containedIn(node keys, [lowestKey, highestKey)) && overlap(node box, search box), flags the node
+// when it returns true and descends into the node's children only then. So leaf l flags node n iff overlaps() holds
+// for l at n and at every ancestor of n; leaves whose search box lies inside the local key range walk nothing.
+//
+// GPU design: one wave owns 64 consecutive local leaves, which are close on the SFC and share most of their walks.
+// The wave walks the union depth-first with an explicit stack in LDS; expanding a node tests its 8 children at once
+// (their boxes, prefixes and child offsets are contiguous and wave-uniform) for the lanes that passed the node, and
+// keeps the per-child ballot of passing lanes. A child with a non-empty ballot is flagged (one byte store per node and
+// wave instead of one per leaf) and, if internal, expanded with that ballot. Each lane is tested exactly where its own
+// reference walk tests, so the flag set is the reference's. Every internal node is expanded at most once per wave,
+// and the stack depth is bounded by the key's level count.
+constexpr int kHaloThreads  = 256;
+constexpr int kHaloWaves    = kHaloThreads / kWave;
+constexpr int kHaloMaxDepth = 22;
 
+struct HaloShared
+{
+    uint64_t mask[kHaloWaves][kHaloMaxDepth][8];
+    int32_t cco[kHaloWaves][kHaloMaxDepth][8]; // first child of each of the frame's 8 nodes
+    int32_t co[kHaloWaves][kHaloMaxDepth];
+    int32_t next[kHaloWaves][kHaloMaxDepth];
+};
+
+template<class K, class T>
+__global__ __launch_bounds__(kHaloThreads) void findHalosKernel(const K* __restrict__ prefixes,
+                                                                const int32_t* __restrict__ childOffsets,
+                                                                const T* __restrict__ nodeCenters,
+                                                                const T* __restrict__ nodeSizes,
+                                                                const K* __restrict__ leaves,
+                                                                const T* __restrict__ searchCenters,
+                                                                const T* __restrict__ searchSizes,
+                                                                typename BoxOf<T>::type box, int32_t firstNode,
+                                                                int32_t lastNode, int32_t numNodes, uint8_t* flags,
+                                                                uint32_t numBlocksPadded)
+{
+    __shared__ HaloShared sh;
+    const int lane       = laneId();
+    const int w          = threadIdx.x / kWave;
+    const uint32_t block = xcdContiguousBlock(blockIdx.x, numBlocksPadded);
+    const int64_t base   = int64_t(firstNode) + int64_t(uniform(block * uint32_t(kHaloWaves) + uint32_t(w))) * kWave;
+    if (base >= lastNode) return;
+    const int32_t leafIdx = int32_t(base) + lane;
+    const K lowestKey     = leaves[firstNode];
+    const K highestKey    = leaves[lastNode];
+
+    T tC[3] = {0, 0, 0}, tS[3] = {0, 0, 0};
+    bool active = leafIdx < lastNode;
+    if (active)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            tC[d] = searchCenters[size_t(leafIdx) * 3 + d];
+            tS[d] = searchSizes[size_t(leafIdx) * 3 + d];
+        }
+        active = !boxContainedInRange<K, T>(lowestKey, highestKey, tC, tS, box);
+    }
     const bool pbc[3] = {box.bnd[0] == HIPCSTONE_PERIODIC, box.bnd[1] == HIPCSTONE_PERIODIC,
                          box.bnd[2] == HIPCSTONE_PERIODIC};
-    auto overlaps = [&](int32_t idx) -> bool {
-        K pfx        = prefixes[idx];
-        unsigned pl  = decodePrefixLength(pfx);
-        K ret        = pfx ^ (K(1) << pl);
-        int ns       = 3 * int(maxLevel<K>()) - int(pl);
-        K nk1        = ret << ns;
-        K nk2        = nk1 + (K(1) << ns);
-        bool contained = !(nk1 < lowestKey || nk2 > highestKey);
-        if (contained) return false;
+
+    // containedIn(nodeStart, nodeEnd, lowestKey, highestKey) of the node with this prefix (boxoverlap.hpp:163-168)
+    auto excluded = [&](K pfx) -> bool {
+        unsigned pl = decodePrefixLength(pfx);
+        K ret       = pfx ^ (K(1) << pl);
+        int ns      = 3 * int(maxLevel<K>()) - int(pl);
+        K nk1       = ret << ns;
+        K nk2       = nk1 + (K(1) << ns);
+        return !(nk1 < lowestKey || nk2 > highestKey);
+    };
+    // overlap(node box, search box) with the reference's operation order (boxoverlap.hpp:280-291)
+    auto overlapBox = [&](const T* c, const T* s) -> bool {
         bool ov = true;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
         {
-            T dx = tC[d] - nodeCenters[size_t(idx) * 3 + d];
+            T dx = tC[d] - c[d];
             if (pbc[d]) dx = dx - box.len[d] * rint(dx * box.ilen[d]);
             dx = fabs(dx);
-            dx -= nodeSizes[size_t(idx) * 3 + d];
+            dx -= s[d];
             dx -= tS[d];
             ov = ov && dx < T(0);
         }
-        if (ov) flags[idx] = 1;
         return ov;
     };
 
-    if (!overlaps(0)) return;
-    if (childOffsets[0] == 0) return;
-    int32_t node      = childOffsets[0];
-    bool backtrack    = false;
+    // test the 8 children of a node for the lanes in pm, flag those some lane passes, push the internal ones
+    auto pushFrame = [&](int depth, int32_t co, uint64_t pm) {
+        const bool eligible = (pm >> lane) & 1u;
+        const T* c          = nodeCenters + size_t(co) * 3;
+        const T* s          = nodeSizes + size_t(co) * 3;
+        T cv[24], sv[24];
+#pragma unroll
+        for (int k = 0; k < 24; ++k)
+        {
+            cv[k] = c[k];
+            sv[k] = s[k];
+        }
+        uint64_t mk[8];
+        int32_t cco[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            bool ex = excluded(prefixes[co + k]);
+            cco[k]  = childOffsets[co + k];
+            mk[k]   = ballot(eligible && !ex && overlapBox(cv + 3 * k, sv + 3 * k));
+        }
+        if (lane < 8)
+        {
+            uint64_t mine = mk[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k)
+                mine = lane == k ? mk[k] : mine;
+            if (mine != 0) flags[co + lane] = 1;
+        }
+        if (lane == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+            {
+                sh.mask[w][depth][k] = cco[k] != 0 ? mk[k] : 0;
+                sh.cco[w][depth][k]  = cco[k];
+            }
+            sh.co[w][depth]   = co;
+            sh.next[w][depth] = 0;
+        }
+        waveSync();
+    };
+
+    // root (collisions.hpp:33-34 via singleTraversal, traversal.hpp:35-45)
+    uint64_t rootMask = ballot(active && !excluded(prefixes[0]) && overlapBox(nodeCenters, nodeSizes));
+    if (rootMask == 0) return;
+    if (lane == 0) flags[0] = 1;
+    int32_t rootChild = uniform(childOffsets[0]);
+    if (rootChild == 0) return;
+    if (rootChild < 1 || rootChild + 8 > numNodes) return; // malformed tree guard
+    int depth = 0;
+    pushFrame(0, rootChild, rootMask);
     int64_t stepsLeft = 2 * int64_t(numNodes) + 8;
-    while (node != 0 && stepsLeft-- > 0)
+    while (depth >= 0 && stepsLeft-- > 0)
     {
-        if (node < 0 || node >= numNodes) break;
-        bool isLeaf  = childOffsets[node] == 0;
-        bool descend = !backtrack && overlaps(node);
-        int32_t sib  = (node - 1) % 8;
-        if (!isLeaf && descend)
+        int k      = uniform(sh.next[w][depth]);
+        uint64_t m = 0;
+        while (k < 8)
         {
-            node      = childOffsets[node];
-            backtrack = false;
+            m = uniform64(sh.mask[w][depth][k]);
+            if (m != 0) break;
+            ++k;
         }
-        else if (sib < 7)
+        if (k >= 8)
         {
-            node++;
-            backtrack = false;
+            --depth;
+            continue;
         }
-        else
-        {
-            node      = parents[(node - 1) / 8];
-            backtrack = true;
-        }
+        int32_t node = uniform(sh.co[w][depth]) + k;
+        int32_t cco  = uniform(sh.cco[w][depth][k]);
+        if (lane == 0) sh.next[w][depth] = k + 1;
+        waveSync();
+        if (node <= 0 || node >= numNodes) break; // malformed tree guard
+        if (depth + 1 >= kHaloMaxDepth || cco <= node || cco + 8 > numNodes) break;
+        ++depth;
+        pushFrame(depth, cco, m);
     }
 }
 
@@ -198,9 +284,13 @@ static hipcstone_status findHalos(hipcstone_stream s, const K* prefixes, const i
     if (!prefixes || !childOffsets || !parents || !nodeCenters || !nodeSizes || !leaves || !searchCenters ||
         !searchSizes || !box || !flags)
         return invalid("findHalos: null pointer");
-    findHalosKernel<K, T><<<ceilDiv(size_t(lastNode - firstNode), 128), 128, 0, stream(s)>>>(
-        prefixes, childOffsets, parents, nodeCenters, nodeSizes, leaves, searchCenters, searchSizes, *box, firstNode,
-        lastNode, numNodes, flags);
+    if (numNodes <= 0) return invalid("findHalos: numNodes must be positive");
+    size_t groups   = (size_t(lastNode - firstNode) + kWave - 1) / kWave;
+    unsigned blocks = ceilDiv(groups * kWave, kHaloThreads);
+    blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
+    findHalosKernel<K, T><<<blocks, kHaloThreads, 0, stream(s)>>>(prefixes, childOffsets, nodeCenters, nodeSizes,
+                                                                  leaves, searchCenters, searchSizes, *box, firstNode,
+                                                                  lastNode, numNodes, flags, blocks);
     HC_CHECK_LAUNCH("findHalosKernel");
     return HIPCSTONE_OK;
 }

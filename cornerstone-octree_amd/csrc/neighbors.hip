@@ -53,16 +53,6 @@ constexpr SlotTables makeSlotTables()
 
 __constant__ SlotTables kSlotTables = makeSlotTables();
 
-// XCD-aware work order: workgroups are dealt round-robin to the 8 XCDs (each with its own L2); remap so that each
-// XCD walks one contiguous range of groups, whose neighbouring targets share leaves and tree nodes in its L2
-constexpr uint32_t kXcds = 8;
-
-__device__ inline uint32_t xcdContiguousBlock(uint32_t b, uint32_t numBlocksPadded)
-{
-    uint32_t perXcd = numBlocksPadded / kXcds;
-    return (b % kXcds) * perXcd + b / kXcds;
-}
-
 template<class T>
 struct alignas(4 * sizeof(T)) Particle4
 {
@@ -89,21 +79,6 @@ struct NbShared
     TargetRec<T> target[kNbWaves][kWave]; // per lane: x, y, z, 4h^2, neighbors so far, pbc flags, row offset
     uint8_t active[kNbWaves][2 * kWave];  // compacted active target lanes of the current leaf (+ zero padding)
 };
-
-// order LDS writes of some lanes before LDS reads of other lanes of the same wave
-__device__ inline void waveSync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline uint64_t uniform64(uint64_t v)
-{
-    uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
-    uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
-    return (uint64_t(hi) << 32) | lo;
-}
 
 template<class T, bool AnyPbc>
 __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(

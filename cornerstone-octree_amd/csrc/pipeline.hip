@@ -19,133 +19,16 @@
 
 #include "common.hpp"
 #include "runtime.hpp"
+#include "host_util.hpp"
 #include "scan.hpp"
 
 
 namespace hipcstone
 {
 
-struct DeviceBuf
-{
-    void* ptr    = nullptr;
-    size_t bytes = 0;
-    ~DeviceBuf()
-    {
-        if (ptr) (void)hipFree(ptr);
-    }
-    hipcstone_status reserve(size_t b)
-    {
-        if (b <= bytes) return HIPCSTONE_OK;
-        if (ptr) HC_CHECK(hipFree(ptr));
-        ptr   = nullptr;
-        bytes = 0;
-        size_t grown = b + b / 8 + 256;
-        hipError_t e = hipMalloc(&ptr, grown);
-        if (e != hipSuccess)
-        {
-            ptr = nullptr;
-            setError(std::string("pipeline: hipMalloc failed: ") + hipGetErrorString(e));
-            return HIPCSTONE_OUT_OF_MEMORY;
-        }
-        bytes = grown;
-        return HIPCSTONE_OK;
-    }
-    template<class V>
-    V* as() const
-    {
-        return static_cast<V*>(ptr);
-    }
-};
-
-// HIP events around each stage of a sync step, read back lazily (no synchronization while recording)
-class StageTimer
-{
-public:
-    static constexpr int kStages = HIPCSTONE_PIPELINE_STAGES;
-    ~StageTimer()
-    {
-        for (auto& r : pending_)
-        {
-            (void)hipEventDestroy(r.start);
-            (void)hipEventDestroy(r.stop);
-        }
-        for (auto e : pool_)
-            (void)hipEventDestroy(e);
-    }
-    void enable(bool on)
-    {
-        on_ = on;
-        for (int i = 0; i < kStages; ++i)
-        {
-            ms_[i]  = 0;
-            cnt_[i] = 0;
-        }
-    }
-    hipcstone_status begin(int stage, hipcstone_stream s)
-    {
-        if (!on_) return HIPCSTONE_OK;
-        Rec r{stage, get(), get()};
-        if (!r.start || !r.stop) return invalid("stage timer: event creation failed");
-        HC_CHECK(hipEventRecord(r.start, stream(s)));
-        pending_.push_back(r);
-        return HIPCSTONE_OK;
-    }
-    hipcstone_status end(hipcstone_stream s)
-    {
-        if (!on_ || pending_.empty()) return HIPCSTONE_OK;
-        HC_CHECK(hipEventRecord(pending_.back().stop, stream(s)));
-        return HIPCSTONE_OK;
-    }
-    hipcstone_status collect(double* ms, int32_t* cnt)
-    {
-        for (auto& r : pending_)
-        {
-            HC_CHECK(hipEventSynchronize(r.stop));
-            float t = 0;
-            HC_CHECK(hipEventElapsedTime(&t, r.start, r.stop));
-            ms_[r.stage] += t;
-            cnt_[r.stage] += 1;
-            pool_.push_back(r.start);
-            pool_.push_back(r.stop);
-        }
-        pending_.clear();
-        for (int i = 0; i < kStages; ++i)
-        {
-            if (ms) ms[i] = ms_[i];
-            if (cnt) cnt[i] = cnt_[i];
-        }
-        return HIPCSTONE_OK;
-    }
-
-private:
-    struct Rec
-    {
-        int stage;
-        hipEvent_t start;
-        hipEvent_t stop;
-    };
-    hipEvent_t get()
-    {
-        if (!pool_.empty())
-        {
-            hipEvent_t e = pool_.back();
-            pool_.pop_back();
-            return e;
-        }
-        hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-    bool on_ = false;
-    double ms_[kStages]{};
-    int32_t cnt_[kStages]{};
-    std::vector<Rec> pending_;
-    std::vector<hipEvent_t> pool_;
-};
-
 struct PipelineBase
 {
-    StageTimer timer;
+    StageTimer<HIPCSTONE_PIPELINE_STAGES> timer;
     virtual ~PipelineBase() = default;
     virtual hipcstone_status sync(hipcstone_stream s, const void* x, const void* y, const void* z, const void* h,
                                   size_t n, const void* box, int fullBuild, uint32_t* nb, uint32_t* nc) = 0;
@@ -265,7 +148,7 @@ public:
         T* ho = xyzh_[out][3].template as<T>();
 
         K* keys = keys_.as<K>();
-        StageTimer& tm = this->timer;
+        auto& tm = this->timer;
 #define HC_STAGE(id, call)                                                                                             \
     do {                                                                                                               \
         if (auto e_ = tm.begin(id, s)) return e_;                                                                     \
@@ -319,7 +202,7 @@ public:
         const T* y = xyzh_[cur_][1].template as<T>();
         const T* z = xyzh_[cur_][2].template as<T>();
         const T* h = xyzh_[cur_][3].template as<T>();
-        StageTimer& tm = this->timer;
+        auto& tm = this->timer;
 #define HC_STAGE(id, call)                                                                                             \
     do {                                                                                                               \
         if (auto e_ = tm.begin(id, s)) return e_;                                                                     \

@@ -1,6 +1,6 @@
 // Synthetic-workload helpers for the benchmarks and tests: per-particle random displacement
 // x_i += scale * h_i * u_i with u_i uniform in [-1, 1)^3 from a counter-based hash of (seed, step, i, dim),
-// clamped into the box, and a Gaussian-mixture particle generator with density-matched h. Not part of the octree
+// clamped into the box, and a Gaussian-mixture particle generator with h matched to the mixture density. Not part of the octree
 // path itself; they produce the inputs that the path consumes.
 #include <limits>
 
@@ -51,44 +51,53 @@ __device__ inline double uniformAt(uint64_t seed, uint64_t stream, uint64_t k)
 // Gaussian-mixture particles: `clusters` isotropic Gaussians (centres uniform in [0,1)^3 from streams 10..12, sigma
 // log-uniform in [0.005, 0.05] from stream 13) plus a uniform background fraction (stream 14 selects, streams 20..22
 // place); cluster choice from stream 15, Box-Muller normals from streams 16..19; coordinates clipped to [0, 1).
-// h = 0.5 (target / (4/3 pi rho))^(1/3) with rho the density of the particle's own component (cluster or
-// background) for n_total particles. Particle i is element offset + i of every stream.
+// h = 0.5 (target / (4/3 pi rho))^(1/3) with rho the mixture's density at the particle for n_total particles
+// (background plus every cluster), so that h follows the local density as an SPH smoothing length does.
+// Particle i is element offset + i of every stream. Up to kMaxClusters clusters.
+constexpr int kMaxClusters = 256;
+
 template<class T>
 __global__ __launch_bounds__(256) void gaussianMixtureKernel(T* __restrict__ x, T* __restrict__ y, T* __restrict__ z,
                                                              T* __restrict__ h, size_t n, uint64_t offset,
                                                              uint64_t seed, uint32_t clusters, double background,
                                                              double target, double nTotal)
 {
+    __shared__ double cx[kMaxClusters], cy[kMaxClusters], cz[kMaxClusters], cg[kMaxClusters], cs[kMaxClusters],
+        cn[kMaxClusters];
+    for (uint32_t c = threadIdx.x; c < clusters; c += blockDim.x)
+    {
+        double sig = exp(log(0.005) + (log(0.05) - log(0.005)) * uniformAt(seed, 13, c));
+        cx[c]      = uniformAt(seed, 10, c);
+        cy[c]      = uniformAt(seed, 11, c);
+        cz[c]      = uniformAt(seed, 12, c);
+        cg[c]      = sig;
+        cs[c]      = -0.5 / (sig * sig);
+        cn[c]      = (1.0 - background) / double(clusters) * nTotal / (pow(2.0 * M_PI, 1.5) * sig * sig * sig);
+    }
+    __syncthreads();
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint64_t k = offset + i;
     double p[3];
-    double rho;
     if (uniformAt(seed, 14, k) < background)
     {
         for (int d = 0; d < 3; ++d)
             p[d] = uniformAt(seed, 20 + d, k);
-        rho = background * nTotal;
     }
     else
     {
         uint32_t c = uint32_t(uniformAt(seed, 15, k) * clusters);
         c          = c < clusters - 1 ? c : clusters - 1;
-        double sig = exp(log(0.005) + (log(0.05) - log(0.005)) * uniformAt(seed, 13, c));
+        double sig = cg[c];
         double u1  = uniformAt(seed, 16, k), u2 = uniformAt(seed, 17, k);
         double u3  = uniformAt(seed, 18, k), u4 = uniformAt(seed, 19, k);
         u1         = u1 > 1e-300 ? u1 : 1e-300;
         u3         = u3 > 1e-300 ? u3 : 1e-300;
         double r1  = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
         double g[3] = {r1 * cos(2.0 * M_PI * u2), r1 * sin(2.0 * M_PI * u2), r2 * cos(2.0 * M_PI * u4)};
-        double r2sum = 0;
+        double ctr[3] = {cx[c], cy[c], cz[c]};
         for (int d = 0; d < 3; ++d)
-        {
-            p[d] = uniformAt(seed, 10 + d, c) + sig * g[d];
-            r2sum += sig * sig * g[d] * g[d];
-        }
-        double norm = (1.0 - background) / double(clusters) * nTotal / (pow(2.0 * M_PI, 1.5) * sig * sig * sig);
-        rho         = norm * exp(-0.5 * r2sum / (sig * sig)) + background * nTotal;
+            p[d] = ctr[d] + sig * g[d];
     }
     const T top = T(1) - std::numeric_limits<T>::epsilon() / T(2); // largest T below 1
     T q[3];
@@ -96,6 +105,12 @@ __global__ __launch_bounds__(256) void gaussianMixtureKernel(T* __restrict__ x, 
     {
         T v  = T(p[d] < 0.0 ? 0.0 : p[d]);
         q[d] = v < top ? v : top;
+    }
+    double rho = background * nTotal;
+    for (uint32_t c = 0; c < clusters; ++c)
+    {
+        double dx = double(q[0]) - cx[c], dy = double(q[1]) - cy[c], dz = double(q[2]) - cz[c];
+        rho += cn[c] * exp(cs[c] * (dx * dx + dy * dy + dz * dz));
     }
     double hh = 0.5 * cbrt(target / (4.0 / 3.0 * M_PI * rho));
     x[i]      = q[0];
@@ -111,7 +126,8 @@ static hipcstone_status gaussianMixture(hipcstone_stream s, T* x, T* y, T* z, T*
 {
     if (n == 0) return HIPCSTONE_OK;
     if (!x || !y || !z || !h) return invalid("gaussianMixture: null pointer");
-    if (clusters == 0 || !(background >= 0.0 && background <= 1.0) || !(nTotal > 0.0) || !(target > 0.0))
+    if (clusters == 0 || clusters > uint32_t(kMaxClusters) || !(background >= 0.0 && background <= 1.0) ||
+        !(nTotal > 0.0) || !(target > 0.0))
         return invalid("gaussianMixture: bad parameters");
     gaussianMixtureKernel<T><<<ceilDiv(n, 256), 256, 0, stream(s)>>>(x, y, z, h, n, offset, seed, clusters,
                                                                      background, target, nTotal);

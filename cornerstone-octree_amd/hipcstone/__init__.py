@@ -594,3 +594,85 @@ class Pipeline:
             _lib.hipcstone_pipeline_destroy(self.handle)
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- one-rank domain
+
+
+class DomainView(C.Structure):
+    _fields_ = [
+        ("num_particles", C.c_size_t),
+        ("start_index", C.c_size_t),
+        ("end_index", C.c_size_t),
+        ("num_global_leaves", C.c_int32),
+        ("num_leaves", C.c_int32),
+        ("num_nodes", C.c_int32),
+        ("num_internal", C.c_int32),
+        ("global_steps", C.c_int32),
+        ("focus_steps", C.c_int32),
+        ("focus_converged", C.c_int32),
+        ("box_lo", C.c_double * 3),
+        ("box_hi", C.c_double * 3),
+    ] + [(name, C.c_void_p) for name in (
+        "keys", "global_leaves", "global_counts", "leaves", "counts", "node_counts", "prefixes", "child_offsets",
+        "parents", "internal_to_leaf", "leaf_to_internal", "level_range", "centers", "sizes", "layout")]
+
+
+class Domain:
+    """One-rank Domain::sync (hipcstone_domain_*): fitted box, Hilbert keys, sort, global leaf array, focus tree;
+    sync reorders the particle arrays in place into SFC order."""
+
+    STAGES = ("box", "keys", "sort_gather", "global_tree", "focus_update", "focus_counts", "finish")
+
+    def __init__(self, box, key_bits=64, bucket_size=64, bucket_size_focus=16, theta=0.5):
+        self.handle = C.c_void_p()
+        self.key_dtype = np.uint64 if key_bits == 64 else np.uint32
+        self.real_dtype = np.dtype(box_dtype(box))
+        self.max_level = 21 if key_bits == 64 else 10
+        real_bits = 32 if self.real_dtype == np.float32 else 64
+        _check(_lib.hipcstone_domain_create(C.byref(self.handle), C.c_int(key_bits), C.c_int(real_bits),
+                                            C.c_uint32(bucket_size), C.c_uint32(bucket_size_focus), C.c_float(theta),
+                                            C.byref(box)), "domain_create")
+
+    def sync(self, x, y, z, h, n=None, keys_out=None, stream=None):
+        n = x.count if n is None else n
+        _check(_lib.hipcstone_domain_sync(self.handle, _stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_size_t(n),
+                                          _p(keys_out)), "domain_sync")
+
+    def view(self) -> DomainView:
+        v = DomainView()
+        _check(_lib.hipcstone_domain_get_view(self.handle, C.byref(v)), "domain_get_view")
+        return v
+
+    def set_timing(self, enable=True):
+        _check(_lib.hipcstone_domain_set_timing(self.handle, C.c_int(int(enable))), "domain_set_timing")
+
+    def stage_times(self):
+        ms = (C.c_double * len(self.STAGES))()
+        cnt = (C.c_int32 * len(self.STAGES))()
+        _check(_lib.hipcstone_domain_stage_times(self.handle, ms, cnt), "domain_stage_times")
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(self.STAGES)}
+
+    def arrays(self):
+        v = self.view()
+        n, G, L, N = v.num_particles, v.num_global_leaves, v.num_leaves, v.num_nodes
+        kd, T = self.key_dtype, self.real_dtype
+        return dict(
+            keys=download_ptr(v.keys, n, kd), global_leaves=download_ptr(v.global_leaves, G + 1, kd),
+            global_counts=download_ptr(v.global_counts, G, np.uint32), leaves=download_ptr(v.leaves, L + 1, kd),
+            counts=download_ptr(v.counts, L, np.uint32), node_counts=download_ptr(v.node_counts, N, np.uint32),
+            prefixes=download_ptr(v.prefixes, N, kd), childOffsets=download_ptr(v.child_offsets, N, np.int32),
+            parents=download_ptr(v.parents, max(1, (N - 1) // 8), np.int32),
+            internalToLeaf=download_ptr(v.internal_to_leaf, N, np.int32),
+            leafToInternal=download_ptr(v.leaf_to_internal, N, np.int32),
+            levelRange=download_ptr(v.level_range, self.max_level + 2, np.int32),
+            centers=download_ptr(v.centers, 3 * N, T).reshape(N, 3), sizes=download_ptr(v.sizes, 3 * N, T).reshape(N, 3),
+            layout=download_ptr(v.layout, L + 1, np.uint32),
+            box_lo=list(v.box_lo), box_hi=list(v.box_hi),
+        )
+
+    def __del__(self):
+        try:
+            _lib.hipcstone_domain_destroy(self.handle)
+        except Exception:
+            pass

@@ -434,6 +434,63 @@ hipcstone_status hipcstone_pipeline_halos(hipcstone_pipeline* p, hipcstone_strea
 hipcstone_status hipcstone_pipeline_set_timing(hipcstone_pipeline* p, int enable);
 hipcstone_status hipcstone_pipeline_stage_times(hipcstone_pipeline* p, double* ms, int32_t* launches);
 
+/* ---- a18: one-rank Domain::sync. Replaces Domain<KeyType, T, Gpu>(rank 0, 1 rank, bucketSize, bucketSizeFocus,
+ * theta, box) and its sync(keys, x, y, z, h, {}, scratch), domain/domain.hpp:169-218: the box is fitted to the
+ * particles (periodic / fixed dimensions keep the given limits, cubic_open widens to the largest side;
+ * sfc/box_mpi.hpp:51-105) and, after the first call, limited to shrink by at most 5 % per side (sfc/box.hpp:398-415);
+ * Hilbert keys, stable sort, particles reordered in place into SFC order; the global leaf array follows the update
+ * rule of domain/assignment.hpp:115-123 (root start with counts bucketSize-1; repeat while the largest count exceeds
+ * bucketSize on the first call or when a step leaves a count of 8*bucketSize or more); the focus tree
+ * (bucketSizeFocus) starts from the root with count bucketSizeFocus+1, is updated until converged on the first call
+ * and once per later call (octree_focus_mpi.hpp:585-603, domain.hpp:200-203) with every global leaf key mandatory.
+ * With one rank there are no halos: start_index = 0, end_index = n, and the particle layout is the scan of the focus
+ * leaf counts. theta is accepted for interface parity; it decides nothing at one rank. box points to a
+ * hipcstone_box_f32/f64 matching real_bits. keys_out (optional, n keys) receives the sorted keys. ---- */
+typedef struct hipcstone_domain hipcstone_domain;
+
+typedef struct
+{
+    size_t num_particles;
+    size_t start_index; /* first assigned particle (Domain::startIndex) */
+    size_t end_index;   /* one past the last (Domain::endIndex) */
+    int32_t num_global_leaves;
+    int32_t num_leaves; /* focus tree */
+    int32_t num_nodes;
+    int32_t num_internal;
+    int32_t global_steps;    /* global update steps of the last sync */
+    int32_t focus_steps;     /* focus updates of the last sync */
+    int32_t focus_converged; /* the last focus update left the tree unchanged */
+    double box_lo[3];        /* box of the last sync */
+    double box_hi[3];
+    void* keys;          /* sorted keys (K) */
+    void* global_leaves; /* global leaf array, num_global_leaves + 1 (K) */
+    void* global_counts; /* u32 */
+    void* leaves;        /* focus leaves, num_leaves + 1 (K) */
+    void* counts;        /* focus leaf counts (u32) */
+    void* node_counts;   /* focus node counts, num_nodes (u32) */
+    void* prefixes;      /* linked focus tree, as hipcstone_build_octree_* returns it */
+    void* child_offsets;
+    void* parents;
+    void* internal_to_leaf;
+    void* leaf_to_internal;
+    void* level_range;
+    void* centers; /* geometric node centers / sizes, Vec3 per node (T) */
+    void* sizes;
+    void* layout; /* first particle of each focus leaf, num_leaves + 1 (u32) */
+} hipcstone_domain_view;
+
+hipcstone_status hipcstone_domain_create(hipcstone_domain** d, int key_bits, int real_bits, uint32_t bucket_size,
+                                         uint32_t bucket_size_focus, float theta, const void* box);
+hipcstone_status hipcstone_domain_destroy(hipcstone_domain* d);
+hipcstone_status hipcstone_domain_sync(hipcstone_domain* d, hipcstone_stream s, void* x, void* y, void* z, void* h,
+                                       size_t n, void* keys_out);
+hipcstone_status hipcstone_domain_get_view(const hipcstone_domain* d, hipcstone_domain_view* v);
+/* per-stage timing as for the pipeline; stages: 0 box, 1 keys, 2 sort+gather, 3 global tree, 4 focus update,
+ * 5 focus counts, 6 centers + layout + copy-out */
+#define HIPCSTONE_DOMAIN_STAGES 7
+hipcstone_status hipcstone_domain_set_timing(hipcstone_domain* d, int enable);
+hipcstone_status hipcstone_domain_stage_times(hipcstone_domain* d, double* ms, int32_t* launches);
+
 /* ---- synthetic workload helper (not an octree stage): x += scale*h*u, u uniform in [-1,1)^3 from a counter-based
  * hash of (seed, step, particle, dimension), clamped into the box; the moving input of multi-step runs ---- */
 hipcstone_status hipcstone_synth_displace_f32(hipcstone_stream s, float* x, float* y, float* z, const float* h,
@@ -443,11 +500,11 @@ hipcstone_status hipcstone_synth_displace_f64(hipcstone_stream s, double* x, dou
                                               size_t n, double scale, uint64_t seed, uint64_t step,
                                               const hipcstone_box_f64* box);
 
-/* ---- synthetic workload helper (not an octree stage): Gaussian-mixture particles in [0,1)^3 — `clusters`
- * isotropic Gaussians (centres uniform, sigma log-uniform in [0.005, 0.05]) plus a uniform `background` fraction,
- * clipped to [0,1) — with h = 0.5 (target_neighbors / (4/3 pi rho))^(1/3) from the density rho of each particle's
- * own component for n_total particles. Particle i is element offset + i of counter-based splitmix64 streams, so a
- * large set can be made in slices. ---- */
+/* ---- synthetic workload helper (not an octree stage): Gaussian-mixture particles in [0,1)^3 — `clusters` (at most
+ * 256) isotropic Gaussians (centres uniform, sigma log-uniform in [0.005, 0.05]) plus a uniform `background`
+ * fraction, clipped to [0,1) — with h = 0.5 (target_neighbors / (4/3 pi rho))^(1/3) from the mixture's density rho
+ * at the particle (background plus all clusters) for n_total particles. Particle i is element offset + i of
+ * counter-based splitmix64 streams, so a large set can be made in slices. ---- */
 hipcstone_status hipcstone_synth_gaussian_mixture_f32(hipcstone_stream s, float* x, float* y, float* z, float* h,
                                                       size_t n, uint64_t offset, uint64_t seed, uint32_t clusters,
                                                       double background, double target_neighbors, double n_total);

@@ -226,3 +226,4 @@ void orc_gather_leaf_ranges(const uint32_t* srcLayout, const uint32_t* dstLayout
 }
 
 void orc_free(void* p) { free(p); }
+void* orc_malloc(size_t bytes) { return malloc(bytes > 0 ? bytes : 1); }

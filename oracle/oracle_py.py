@@ -61,6 +61,8 @@ class CpuLib:
         self.kind = kind
         self.lib = C.CDLL(path)
         self.pre = "orc_" if kind == "oracle" else "ref_"
+        if kind == "oracle":
+            self.lib.orc_malloc.restype = C.c_void_p
 
     def _f(self, name):
         return getattr(self.lib, self.pre + name)
@@ -346,6 +348,105 @@ class CpuLib:
             C.c_int(0 if accumulate_flags is None else 1))
         return sc, ss, flags
 
+    # ---- focus tree and the domain's global-tree rule
+    def _max_key(self, kd):
+        return np.array(1 << (63 if np.dtype(kd) == np.uint64 else 30), dtype=kd)
+
+    def update_focus(self, leaves, bucket, focus_start, focus_end, mandatory, counts, macs=None):
+        """CombinedUpdate::updateFocus (octree_focus.hpp:64-111): (new leaves, converged)."""
+        kd = leaves.dtype
+        mandatory = np.ascontiguousarray(mandatory, dtype=kd)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        macs = None if macs is None else np.ascontiguousarray(macs, dtype=np.uint8)
+        L = leaves.size - 1
+        kt = C.c_uint32 if kd == np.uint32 else C.c_uint64
+        if self.kind == "oracle":
+            p = C.c_void_p(self.lib.orc_malloc(C.c_size_t(leaves.nbytes)))
+            C.memmove(p, leaves.ctypes.data, leaves.nbytes)
+            nl = C.c_int(L)
+            f = self._f(f"update_focus_{_KS[kd]}")
+            conv = f(C.byref(p), C.byref(nl), C.c_uint(bucket), kt(int(focus_start)), kt(int(focus_end)),
+                     _ptr(mandatory), C.c_int(mandatory.size), _ptr(counts), _ptr(macs))
+            out = np.ctypeslib.as_array(C.cast(p, C.POINTER(kt)), shape=(nl.value + 1,)).copy()
+            self.lib.orc_free(p)
+            return out, bool(conv)
+        cap = max(64, 16 * (L + 1) + 8 * mandatory.size)
+        while True:
+            buf = np.zeros(cap + 1, dtype=kd)
+            buf[: L + 1] = leaves
+            nl = C.c_int(L)
+            conv = self._f(f"update_focus_{_KS[kd]}")(_ptr(buf), C.byref(nl), C.c_int(cap), C.c_uint(bucket),
+                                                      kt(int(focus_start)), kt(int(focus_end)), _ptr(mandatory),
+                                                      C.c_int(mandatory.size), _ptr(counts), _ptr(macs))
+            if nl.value <= cap:
+                return buf[: nl.value + 1].copy(), bool(conv)
+            cap = nl.value + 1
+
+    def focus_counts(self, leaves, keys):
+        """Leaf counts and upswept node counts of a focus tree (octree_focus.hpp:264-270)."""
+        L = leaves.size - 1
+        N = L + (L - 1) // 7
+        lc = np.zeros(L, dtype=np.uint32)
+        nc = np.zeros(N, dtype=np.uint32)
+        keys = np.ascontiguousarray(keys)
+        self._f(f"focus_counts_{_KS[leaves.dtype]}")(_ptr(np.ascontiguousarray(leaves)), C.c_int(L), _ptr(keys),
+                                                       C.c_size_t(keys.size), _ptr(lc), _ptr(nc))
+        return lc, nc
+
+    def compute_spanning_tree(self, keys):
+        keys = np.ascontiguousarray(keys)
+        kd = keys.dtype
+        if self.kind == "oracle":
+            f = self.lib[f"orc_compute_spanning_tree_{_KS[kd]}"]
+            f.restype = C.c_void_p
+            n = C.c_int()
+            p = f(_ptr(keys), C.c_int(keys.size), C.byref(n))
+            kt = C.c_uint32 if kd == np.uint32 else C.c_uint64
+            out = np.ctypeslib.as_array(C.cast(p, C.POINTER(kt)), shape=(n.value,)).copy()
+            self.lib.orc_free(C.c_void_p(p))
+            return out
+        cap = 1024
+        while True:
+            out = np.zeros(cap, dtype=kd)
+            n = self._f(f"compute_spanning_tree_{_KS[kd]}")(_ptr(keys), C.c_int(keys.size), _ptr(out), C.c_int(cap))
+            if n <= cap:
+                return out[:n].copy()
+            cap = n
+
+    def domain_global_update(self, sorted_keys, bucket, tree, counts, first_call):
+        """The domain's global-tree rule at one rank (assignment.hpp:115-123): (tree, counts, steps)."""
+        kd = tree.dtype
+        L = tree.size - 1
+        if self.kind == "oracle":
+            kt = C.c_uint32 if kd == np.uint32 else C.c_uint64
+            tp = C.c_void_p(self.lib.orc_malloc(C.c_size_t(tree.nbytes)))
+            C.memmove(tp, tree.ctypes.data, tree.nbytes)
+            c = np.ascontiguousarray(counts, dtype=np.uint32)
+            cp = C.c_void_p(self.lib.orc_malloc(C.c_size_t(max(4, c.nbytes))))
+            C.memmove(cp, c.ctypes.data, c.nbytes)
+            nn = C.c_int(L)
+            steps = self._f(f"domain_global_update_{_KS[kd]}")(_ptr(sorted_keys), C.c_size_t(sorted_keys.size),
+                                                               C.c_uint(bucket), C.byref(tp), C.byref(nn),
+                                                               C.byref(cp), C.c_int(int(first_call)))
+            t = np.ctypeslib.as_array(C.cast(tp, C.POINTER(kt)), shape=(nn.value + 1,)).copy()
+            cc = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_uint32)), shape=(nn.value,)).copy()
+            self.lib.orc_free(tp)
+            self.lib.orc_free(cp)
+            return t, cc, steps
+        cap = max(4096, 8 * (L + 1))
+        while True:
+            t = np.zeros(cap + 1, dtype=kd)
+            t[: L + 1] = tree
+            cc = np.zeros(cap, dtype=np.uint32)
+            cc[:L] = counts
+            nn = C.c_int(L)
+            steps = self._f(f"domain_global_update_{_KS[kd]}")(_ptr(sorted_keys), C.c_size_t(sorted_keys.size),
+                                                               C.c_uint(bucket), _ptr(t), C.byref(nn), _ptr(cc),
+                                                               C.c_int(cap), C.c_int(int(first_call)))
+            if nn.value <= cap:
+                return t[: nn.value + 1].copy(), cc[: nn.value].copy(), steps
+            cap = nn.value
+
     def omp_threads(self):
         assert self.kind == "ref"
         return int(self.lib.ref_omp_threads())
@@ -353,3 +454,100 @@ class CpuLib:
 
 ORACLE = CpuLib(ORACLE_PATH, "oracle") if os.path.exists(ORACLE_PATH) else None
 REF = CpuLib(REF_PATH, "ref") if os.path.exists(REF_PATH) else None
+
+
+def fitting_box(x, y, z, prev_lo, prev_hi, boundary):
+    """makeGlobalBox (sfc/box_mpi.hpp:51-105) at one rank: periodic and fixed dimensions keep the previous limits,
+    the others take the coordinates' min/max; cubic_open dimensions are widened to the largest side. Limits in T."""
+    T = np.asarray(x).dtype.type
+    lo = [T(v) for v in prev_lo]
+    hi = [T(v) for v in prev_hi]
+    for d, a in enumerate((x, y, z)):
+        if boundary[d] not in (1, 2) and a.size:
+            lo[d], hi[d] = T(a.min()), T(a.max())
+    side = max(T(hi[0] - lo[0]), T(hi[1] - lo[1]), T(hi[2] - lo[2]))
+    for d in range(3):
+        if boundary[d] == 3:
+            hi[d] = max(hi[d], T(lo[d] + side))
+    return lo, hi
+
+
+def limit_box_shrinking(fit_lo, fit_hi, prev_lo, prev_hi):
+    """limitBoxShrinking (sfc/box.hpp:398-415) with shrinkLimit 0.05, in T."""
+    T = type(fit_lo[0])
+    s = T(0.05)
+    lo, hi = [], []
+    for d in range(3):
+        L = T(prev_hi[d] - prev_lo[d])
+        lo.append(min(fit_lo[d], T(prev_lo[d] + T(s * L))))
+        hi.append(max(fit_hi[d], T(prev_hi[d] - T(s * L))))
+    return lo, hi
+
+
+class CpuDomain:
+    """Domain<KeyType, T>::sync at one rank (domain/domain.hpp:169-218), sequenced over a CPU lib's functions:
+    fitted / shrink-limited box (assignment.hpp:101-103), Hilbert keys and a stable sort, the global tree rule
+    (assignment.hpp:115-123) from the initial root {0, nodeRange(0)} with counts bucketSize-1 (assignment.hpp:62-66,
+    the spanning tree of initialDomainSplits at one rank), particles gathered into SFC order, then the focus tree
+    (octree_focus_mpi.hpp:61-87: root with count bucketSizeFocus+1): converged on the first call
+    (:585-603), one update per call (domain.hpp:200-203). The focus area is the whole key range and every global
+    leaf key is mandatory (octree_focus_mpi.hpp:120-121), so MAC flags do not enter the decisions."""
+
+    def __init__(self, lib, key_dtype, real_dtype, bucket, bucket_focus, lo, hi, boundary=(0, 0, 0)):
+        self.lib = lib
+        self.kd = np.dtype(key_dtype)
+        self.T = np.dtype(real_dtype).type
+        self.bucket, self.bucket_focus = bucket, bucket_focus
+        self.lo = [self.T(v) for v in np.broadcast_to(lo, (3,))]
+        self.hi = [self.T(v) for v in np.broadcast_to(hi, (3,))]
+        self.boundary = tuple(boundary)
+        top = 1 << (63 if self.kd == np.uint64 else 30)
+        self.top = top
+        self.gtree = np.array([0, top], dtype=self.kd)
+        self.gcounts = np.array([bucket - 1], dtype=np.uint32)
+        self.fleaves = np.array([0, top], dtype=self.kd)
+        self.fnodecounts = np.array([bucket_focus + 1], dtype=np.uint32)
+        self.fleafcounts = self.fnodecounts.copy()
+        self.first = True
+
+    def box(self):
+        return make_box(self.lo, self.hi, self.T, self.boundary)
+
+    def sync(self, x, y, z, h):
+        lo, hi = fitting_box(x, y, z, self.lo, self.hi, self.boundary)
+        if not self.first:
+            if isinstance(self.lib, CpuLib) and self.lib.kind == "ref":
+                out = np.zeros(6, dtype=self.T)
+                fit = np.array([lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]], dtype=self.T)
+                prev = np.array([self.lo[0], self.hi[0], self.lo[1], self.hi[1], self.lo[2], self.hi[2]],
+                                dtype=self.T)
+                bnd = np.array(self.boundary, dtype=np.int32)
+                self.lib._f(f"limit_box_shrinking_{_RS[np.dtype(self.T)]}")(_ptr(fit), _ptr(prev), _ptr(bnd),
+                                                                             _ptr(out))
+                lo, hi = [out[0], out[2], out[4]], [out[1], out[3], out[5]]
+            else:
+                lo, hi = limit_box_shrinking(lo, hi, self.lo, self.hi)
+        self.lo, self.hi = [self.T(v) for v in lo], [self.T(v) for v in hi]
+        box = self.box()
+        keys = self.lib.compute_sfc_keys(x, y, z, box, self.kd, HILBERT)
+        keys, perm = self.lib.sort_by_key(keys)
+        self.gtree, self.gcounts, gsteps = self.lib.domain_global_update(keys, self.bucket, self.gtree, self.gcounts,
+                                                                         self.first)
+        xs, ys, zs, hs = x[perm], y[perm], z[perm], h[perm]
+        fsteps = 0
+        if self.first:
+            while True:
+                fsteps += 1
+                self.fleaves, conv = self.lib.update_focus(self.fleaves, self.bucket_focus, 0, self.top, self.gtree,
+                                                           self.fnodecounts)
+                self.fleafcounts, self.fnodecounts = self.lib.focus_counts(self.fleaves, keys)
+                if conv:
+                    break
+        fsteps += 1
+        self.fleaves, conv = self.lib.update_focus(self.fleaves, self.bucket_focus, 0, self.top, self.gtree,
+                                                   self.fnodecounts)
+        self.fleafcounts, self.fnodecounts = self.lib.focus_counts(self.fleaves, keys)
+        self.first = False
+        return dict(keys=keys, perm=perm, x=xs, y=ys, z=zs, h=hs, lo=self.lo, hi=self.hi, gtree=self.gtree,
+                    gcounts=self.gcounts, gsteps=gsteps, fleaves=self.fleaves, fcounts=self.fleafcounts,
+                    fnodecounts=self.fnodecounts, fsteps=fsteps, converged=conv)

@@ -16,6 +16,7 @@
 #include "cstone/domain/domaindecomp.hpp"
 #include "cstone/domain/layout.hpp"
 #include "cstone/findneighbors.hpp"
+#include "cstone/focus/octree_focus.hpp"
 #include "cstone/focus/source_center.hpp"
 #include "cstone/primitives/gather.hpp"
 #include "cstone/sfc/sfc.hpp"
@@ -325,6 +326,107 @@ extern "C"
 
     REF_DOMAIN(uint32_t, u32)
     REF_DOMAIN(uint64_t, u64)
+
+#define REF_FOCUS(K, KS)                                                                                               \
+    int ref_update_focus_##KS(K* leavesIO, int* numLeaves, int capacity, unsigned bucket, K focusStart, K focusEnd,    \
+                              const K* mandatory, int numMandatory, const unsigned* counts, const uint8_t* macs)       \
+    {                                                                                                                  \
+        std::vector<K> leaves(leavesIO, leavesIO + *numLeaves + 1);                                                    \
+        OctreeData<K, execution::Cpu> tree;                                                                            \
+        tree.resize(nNodes(leaves));                                                                                   \
+        updateInternalTree<K>(leaves, tree.data());                                                                    \
+        size_t N = size_t(tree.numNodes);                                                                              \
+        std::vector<uint8_t> zeroMacs(N, 0);                                                                           \
+        bool conv = CombinedUpdate<K>::updateFocus(tree, leaves, bucket, focusStart, focusEnd,                         \
+                                                   std::span<const K>(mandatory, numMandatory),                        \
+                                                   std::span<const unsigned>(counts, N),                               \
+                                                   std::span<const uint8_t>(macs ? macs : zeroMacs.data(), N));        \
+        int nl = int(leaves.size()) - 1;                                                                               \
+        if (nl <= capacity) std::copy(leaves.begin(), leaves.end(), leavesIO);                                         \
+        *numLeaves = nl;                                                                                               \
+        return conv ? 1 : 0;                                                                                           \
+    }                                                                                                                  \
+    void ref_focus_counts_##KS(const K* leavesIn, int L, const K* keys, size_t n, unsigned* leafCounts,               \
+                               unsigned* nodeCounts)                                                                   \
+    {                                                                                                                  \
+        std::vector<K> leaves(leavesIn, leavesIn + L + 1);                                                             \
+        OctreeData<K, execution::Cpu> tree;                                                                            \
+        tree.resize(nNodes(leaves));                                                                                   \
+        updateInternalTree<K>(leaves, tree.data());                                                                    \
+        std::fill(leafCounts, leafCounts + L, 0u);                                                                     \
+        computeNodeCounts<K>(leaves.data(), leafCounts, L, std::span<const K>(keys, n),                               \
+                             std::numeric_limits<unsigned>::max(), true);                                              \
+        std::fill(nodeCounts, nodeCounts + tree.numNodes, 0u);                                                         \
+        scatter(leafToInternal(tree), leafCounts, nodeCounts);                                                         \
+        upsweep(tree.levelRange, tree.childOffsets.data(), nodeCounts, NodeCount<unsigned>{});                         \
+    }                                                                                                                  \
+    int ref_compute_spanning_tree_##KS(const K* keys, int numKeys, K* out, int capacity)                              \
+    {                                                                                                                  \
+        auto t = computeSpanningTree<K>(std::span<const K>(keys, numKeys));                                            \
+        if (int(t.size()) <= capacity) std::copy(t.begin(), t.end(), out);                                             \
+        return int(t.size());                                                                                          \
+    }                                                                                                                  \
+    /* domain/assignment.hpp:115-123 with tree/update_mpi.hpp:64-98 at one rank (all-reduce = identity), built from \
+       the reference's rebalanceDecision / rebalanceTree / computeNodeCounts */                                        \
+    int ref_domain_global_update_##KS(const K* keys, size_t n, unsigned bucket, K* treeIO, int* numNodes,              \
+                                      unsigned* countsIO, int capacity, int firstCall)                                 \
+    {                                                                                                                  \
+        std::vector<K> leaves(treeIO, treeIO + *numNodes + 1);                                                         \
+        std::vector<unsigned> counts(countsIO, countsIO + *numNodes);                                                  \
+        std::span<const K> keyView(keys, n);                                                                           \
+        auto step = [&]() -> unsigned {                                                                                \
+            std::vector<TreeNodeIndex> ops(nNodes(leaves) + 1);                                                        \
+            bool conv = rebalanceDecision(leaves.data(), counts.data(), nNodes(leaves), bucket, ops.data());           \
+            std::vector<K> newLeaves;                                                                                  \
+            rebalanceTree(leaves, newLeaves, ops.data());                                                              \
+            swap(leaves, newLeaves);                                                                                   \
+            counts.resize(nNodes(leaves));                                                                             \
+            computeNodeCounts(leaves.data(), counts.data(), nNodes(leaves), keyView,                                   \
+                              std::numeric_limits<unsigned>::max(), true);                                             \
+            if (conv) return 0u;                                                                                       \
+            return *std::max_element(counts.begin(), counts.end());                                                    \
+        };                                                                                                             \
+        int steps  = 1;                                                                                                \
+        unsigned m = step();                                                                                           \
+        if (firstCall || m >= 8 * bucket)                                                                              \
+        {                                                                                                              \
+            ++steps;                                                                                                   \
+            while (step() > bucket)                                                                                    \
+                ++steps;                                                                                               \
+        }                                                                                                              \
+        int nn = nNodes(leaves);                                                                                       \
+        if (nn <= capacity)                                                                                            \
+        {                                                                                                              \
+            std::copy(leaves.begin(), leaves.end(), treeIO);                                                           \
+            std::copy(counts.begin(), counts.end(), countsIO);                                                         \
+        }                                                                                                              \
+        *numNodes = nn;                                                                                                \
+        return steps;                                                                                                  \
+    }
+
+    REF_FOCUS(uint32_t, u32)
+    REF_FOCUS(uint64_t, u64)
+
+    void ref_limit_box_shrinking_f64(const double* fitting, const double* previous, const int* bnd, double* out)
+    {
+        auto b = [&](const double* l) {
+            return Box<double>(l[0], l[1], l[2], l[3], l[4], l[5], BoundaryType(bnd[0]), BoundaryType(bnd[1]),
+                               BoundaryType(bnd[2]));
+        };
+        auto r = limitBoxShrinking(b(fitting), b(previous));
+        double v[6] = {r.xmin(), r.xmax(), r.ymin(), r.ymax(), r.zmin(), r.zmax()};
+        std::copy(v, v + 6, out);
+    }
+    void ref_limit_box_shrinking_f32(const float* fitting, const float* previous, const int* bnd, float* out)
+    {
+        auto b = [&](const float* l) {
+            return Box<float>(l[0], l[1], l[2], l[3], l[4], l[5], BoundaryType(bnd[0]), BoundaryType(bnd[1]),
+                              BoundaryType(bnd[2]));
+        };
+        auto r = limitBoxShrinking(b(fitting), b(previous));
+        float v[6] = {r.xmin(), r.xmax(), r.ymin(), r.ymax(), r.zmin(), r.zmax()};
+        std::copy(v, v + 6, out);
+    }
 
     void ref_compute_node_layout(const unsigned* counts, const uint8_t* flags, const int32_t* leafToInternal,
                                  int numLeaves, int numNodes, int first, int last, uint32_t* layout)

@@ -64,3 +64,39 @@ def test_errors_are_status_codes_not_exits():
     h = ctypes.c_void_p()
     assert lib.hipcstone_pipeline_create(ctypes.byref(h), 1, 48, 32, 64, 150) == 1
     assert lib.hipcstone_pipeline_create(ctypes.byref(h), 1, 64, 32, 0, 150) == 1
+
+
+def test_struct_layouts_match_the_binding(tmp_path):
+    """The ctypes mirrors of the header's structs have the C compiler's sizes and field offsets."""
+    import hipcstone
+
+    src = tmp_path / "layout.c"
+    fields = [f for f, _ in hipcstone.PipelineView._fields_]
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"hipcstone.h\"\nint main(void){\n"
+                   + 'printf("%zu %zu %zu\\n", sizeof(hipcstone_pipeline_view), sizeof(hipcstone_box_f32), '
+                     "sizeof(hipcstone_box_f64));\n"
+                   + "".join(f'printf("%zu\\n", offsetof(hipcstone_pipeline_view, {f}));\n' for f in fields)
+                   + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    sizes, offsets = [int(v) for v in out[:3]], [int(v) for v in out[3:]]
+    assert sizes == [ctypes.sizeof(hipcstone.PipelineView), ctypes.sizeof(hipcstone.BoxF32),
+                     ctypes.sizeof(hipcstone.BoxF64)]
+    assert offsets == [getattr(hipcstone.PipelineView, f).offset for f in fields]
+
+
+def test_domain_view_layout_matches_the_binding(tmp_path):
+    import hipcstone
+
+    src = tmp_path / "dlayout.c"
+    fields = [f for f, _ in hipcstone.DomainView._fields_]
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"hipcstone.h\"\nint main(void){\n"
+                   + 'printf("%zu\\n", sizeof(hipcstone_domain_view));\n'
+                   + "".join(f'printf("%zu\\n", offsetof(hipcstone_domain_view, {f}));\n' for f in fields)
+                   + "return 0;}\n")
+    exe = tmp_path / "dlayout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(hipcstone.DomainView)
+    assert out[1:] == [getattr(hipcstone.DomainView, f).offset for f in fields]

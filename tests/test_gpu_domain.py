@@ -1,0 +1,106 @@
+"""GPU parity of the one-rank Domain::sync (box, keys, order, global leaf array, focus tree, its octree, centers and
+layout) against the oracle's one-rank domain sequence over several steps, and the neighbor search on the focus
+tree."""
+
+import numpy as np
+import pytest
+
+from helpers import neighbor_rows_equal
+from oracle_py import CpuDomain
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(hip, a):
+    return hip.DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+
+def plummer_input(n, T, seed=31):
+    from hipcstone import inputs
+
+    x, y, z, r = inputs.plummer(n, 0.1, seed=seed, dtype=T)
+    h = inputs.plummer_h(r, n, 0.1, 60, dtype=T)
+    return x, y, z, h
+
+
+@pytest.mark.parametrize("kd,T,bucket,bucket_focus", [(np.uint64, np.float32, 64, 16), (np.uint64, np.float64, 128, 32),
+                                                      (np.uint32, np.float32, 64, 64), (np.uint32, np.float64, 32, 8)])
+@pytest.mark.parametrize("boundary", [(0, 0, 0), (1, 2, 3)])
+def test_domain_sync_steps(hip, oracle, kd, T, bucket, bucket_focus, boundary):
+    n = 120000
+    x, y, z, h = plummer_input(n, T)
+    lo, hi = -1.0, 1.0
+    gbox = hip.make_box(lo, hi, T, boundary)
+    dom = hip.Domain(gbox, key_bits=64 if kd == np.uint64 else 32, bucket_size=bucket,
+                     bucket_size_focus=bucket_focus)
+    ref = CpuDomain(oracle, kd, T, bucket, bucket_focus, lo, hi, boundary)
+    rng = np.random.default_rng(32)
+    d = [dev(hip, a) for a in (x, y, z, h)]
+    keys_out = hip.DeviceArray(n, kd)
+    for step in range(4):
+        print(f"domain step {step}: {np.dtype(kd).name} {np.dtype(T).name} bucket {bucket}/{bucket_focus} "
+              f"boundary {boundary}", flush=True)
+        xs, ys, zs, hs = (a.download() for a in d)
+        o = ref.sync(xs, ys, zs, hs)
+        dom.sync(*d, n, keys_out)
+        g = dom.arrays()
+        v = dom.view()
+        assert g["box_lo"] == [float(t) for t in o["lo"]] and g["box_hi"] == [float(t) for t in o["hi"]]
+        assert np.array_equal(keys_out.download(), o["keys"]) and np.array_equal(g["keys"], o["keys"])
+        for name, a in zip("xyzh", d):
+            assert np.array_equal(a.download(), o[name]), name
+        assert np.array_equal(g["global_leaves"], o["gtree"]) and np.array_equal(g["global_counts"], o["gcounts"])
+        assert v.global_steps == o["gsteps"] and v.focus_steps == o["fsteps"]
+        assert bool(v.focus_converged) == o["converged"]
+        assert np.array_equal(g["leaves"], o["fleaves"])
+        assert np.array_equal(g["counts"], o["fcounts"]) and np.array_equal(g["node_counts"], o["fnodecounts"])
+        t = oracle.build_octree(o["fleaves"])
+        for k in ("prefixes", "childOffsets", "internalToLeaf", "leafToInternal"):
+            assert np.array_equal(g[k], t[k]), k
+        c, s = oracle.node_fp_centers(t["prefixes"], ref.box(), T)
+        assert np.array_equal(g["centers"], c) and np.array_equal(g["sizes"], s)
+        assert np.array_equal(g["layout"], np.concatenate([[0], np.cumsum(o["fcounts"])]).astype(np.uint32))
+        assert (v.start_index, v.end_index) == (0, n)
+        # move the particles for the next step (host-side, identical for both)
+        moved = [(a.download() + rng.normal(0, 0.005, n)).astype(T) for a in d[:3]]
+        for a, m in zip(d[:3], moved):
+            a.upload(np.clip(m, -1, 1).astype(T))
+
+
+def test_domain_neighbors_on_focus_tree(hip, oracle):
+    """findNeighbors over the domain's focus tree (Domain::octreeProperties) equals the oracle's search over the
+    oracle domain's tree."""
+    n, T, kd = 80000, np.float32, np.uint64
+    x, y, z, h = plummer_input(n, T, seed=33)
+    gbox = hip.make_box(-1.0, 1.0, T)
+    dom = hip.Domain(gbox, key_bits=64, bucket_size=64, bucket_size_focus=16)
+    ref = CpuDomain(oracle, kd, T, 64, 16, -1.0, 1.0)
+    d = [dev(hip, a) for a in (x, y, z, h)]
+    o = ref.sync(x, y, z, h)
+    dom.sync(*d, n)
+    v = dom.view()
+    gb = hip.make_box(v.box_lo[:], v.box_hi[:], T)
+    ngmax = 64
+    nb = hip.DeviceArray(n * ngmax, np.uint32)
+    nc = hip.DeviceArray(n, np.uint32)
+    hip.find_neighbors(d[0], d[1], d[2], d[3], 0, n, gb, v.child_offsets, v.parents, v.internal_to_leaf, v.layout,
+                       v.centers, v.sizes, v.num_nodes, ngmax, nb, nc)
+    t = oracle.build_octree(o["fleaves"])
+    c, s = oracle.node_fp_centers(t["prefixes"], ref.box(), T)
+    layout = np.concatenate([[0], np.cumsum(o["fcounts"])]).astype(np.uint32)
+    ids = np.arange(0, n, 7, dtype=np.uint32)
+    nb_o, nc_o = oracle.find_neighbors_subset(o["x"], o["y"], o["z"], o["h"], ids, ref.box(), t, layout, c, s, ngmax)
+    g_nb = nb.download().reshape(n, ngmax)[ids]
+    g_nc = nc.download()[ids]
+    assert neighbor_rows_equal(g_nb, g_nc, nb_o, nc_o, ngmax)
+    assert g_nc.mean() > 20
+
+
+def test_domain_rejects_bad_arguments(hip):
+    gbox = hip.make_box(0.0, 1.0, np.float32)
+    with pytest.raises(hip.HipcstoneError):
+        hip.Domain(gbox, bucket_size=16, bucket_size_focus=64)
+    dom = hip.Domain(gbox)
+    x = hip.DeviceArray(10, np.float32)
+    with pytest.raises(hip.HipcstoneError):
+        dom.sync(x, x, x, None, 10)
