@@ -161,6 +161,106 @@ def halo_bench(args):
         "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()
                    if c > 0},
     }
+    result["cpu_baseline"] = None if args.no_cpu_baseline else halo_cpu_baseline(hip, args.halo_cpu_sample)
+    print(json.dumps(result), flush=True)
+
+
+def halo_cpu_baseline(hip, n):
+    """The reference's CPU rank-share chain (oracle/_ref) on a bounded sample: the same mixture workload generated
+    at n particles (h for ~100 neighbors at that count), the whole chain run once."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_py import MORTON, REF, make_box
+
+    if REF is None:
+        return None
+    d = [hip.DeviceArray(n, np.float64) for _ in range(4)]
+    hip.gaussian_mixture(*d, n, seed=7, clusters=64, background=0.2, target_neighbors=100.0)
+    x, y, z, h = (a.download() for a in d)
+    t0 = time.time()
+    m, t = REF.halo_chain(x, y, z, h, make_box(0.0, 1.0, np.float64), 128, 8, 2, curve=MORTON)
+    log(f"cpu baseline (halo chain, {n} particles): {time.time() - t0:.1f} s")
+    total = float(t.sum())
+    return {
+        "value": round(n / (total / 1e3) / 1e6, 3),
+        "unit": "Mparticles/s",
+        "cores": REF.omp_threads(),
+        "kind": "reference",
+        "sample": (f"the same mixture workload generated at {n} particles, one pass of the whole rank-share chain "
+                   f"through the reference CPU path (oracle/_ref): {total:.0f} ms, {m} local particles"),
+        "stage_ms": {k: round(float(v), 1) for k, v in zip(
+            ("keys", "sort_gather", "tree", "link", "centers_layout", "assignment", "halos", "layout_gather"), t)},
+    }
+
+
+def domain_bench(args):
+    """One-rank Domain::sync + findNeighbors on the focus tree: the same 16,777,216 fp32 Plummer particles (h from
+    the analytic density), Hilbert-64 keys, bucketSize 64 for the global leaf array and 16 for the focus tree, box
+    fitted to the particles; one step = displacement by 1e-3 h + Domain::sync (box, keys, sort, global tree, focus
+    tree update and counts, centers, layout, particles back in SFC order) + findNeighbors (ngmax 150)."""
+    import hipcstone as hip
+    from hipcstone import inputs
+
+    n, K, W, ngmax = args.n, args.steps, args.warmup, args.ngmax
+    T = np.float32
+    a = 0.05
+    x, y, z, r = inputs.plummer(n, a, seed=2024, dtype=T)
+    h = inputs.plummer_h(r, n, a, 100.0, dtype=T)
+    box = hip.make_box(-1.0, 1.0, T)
+    dom = hip.Domain(box, key_bits=64, bucket_size=64, bucket_size_focus=16)
+    d = [hip.DeviceArray.from_numpy(v) for v in (x, y, z, h)]
+    nb = hip.DeviceArray(n * ngmax, np.uint32)
+    nc = hip.DeviceArray(n, np.uint32)
+    nb_stream_events = [hip.Event(), hip.Event()]
+
+    def step(k, timing=None):
+        hip.displace(d[0], d[1], d[2], d[3], n, 1e-3, 7, k, box)
+        dom.sync(*d, n)
+        v = dom.view()
+        gb = hip.make_box(v.box_lo[:], v.box_hi[:], T)
+        if timing is not None:
+            timing[0].record()
+        hip.find_neighbors(d[0], d[1], d[2], d[3], 0, n, gb, v.child_offsets, v.parents, v.internal_to_leaf,
+                           v.layout, v.centers, v.sizes, v.num_nodes, ngmax, nb, nc)
+        if timing is not None:
+            timing[1].record()
+
+    step(0)
+    for k in range(W):
+        step(1 + k)
+    hip.device_sync()
+    events = [hip.Event() for _ in range(K + 1)]
+    dom.set_timing(True)
+    nb_ms = 0.0
+    events[0].record()
+    for k in range(K):
+        step(1 + W + k, nb_stream_events)
+        events[k + 1].record()
+        hip.device_sync()
+        nb_ms += nb_stream_events[0].elapsed_ms(nb_stream_events[1])
+    per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
+    total_ms = float(per_step.sum())
+    stages = dom.stage_times()
+    dom.set_timing(False)
+    v = dom.view()
+    counts = nc.download()
+    result = {
+        "metric": "Mparticles/s one-rank Domain::sync + findNeighbors (16M fp32)",
+        "value": round(n * K / (total_ms / 1e3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(total_ms / K, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": ("one-rank Domain::sync + findNeighbors: 16,777,216 fp32 Plummer particles (a=0.05), fitted box, "
+                         "Hilbert-64 keys, bucketSize 64 / focus 16, 1e-3 h displacement per step, ngmax 150"),
+            "particles": n, "global_leaves": int(v.num_global_leaves), "focus_leaves": int(v.num_leaves),
+            "mean_neighbors": round(float(counts.mean()), 2),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
+                       "p99": round(float(np.percentile(per_step, 99)), 3)},
+        "stages": dict({name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()
+                        if c > 0}, neighbors={"avg_ms": round(nb_ms / K, 4), "launches": K}),
+        "cpu_baseline": None,
+    }
     print(json.dumps(result), flush=True)
 
 
@@ -173,9 +273,12 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "halo"], default="plain",
-                    help="plain: the 16.8M neighbor pipeline (default); halo: the 134M rank-share chain")
+    ap.add_argument("--workload", choices=["plain", "halo", "domain"], default="plain",
+                    help="plain: the 16.8M neighbor pipeline (default); halo: the 134M rank-share chain; "
+                         "domain: one-rank Domain::sync + neighbors at 16.8M")
     ap.add_argument("--n-halo", type=int, default=134217728)
+    ap.add_argument("--halo-cpu-sample", type=int, default=8388608,
+                    help="particles of the CPU baseline sample of the halo workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (reference reading for HBM traffic counters)")
     args = ap.parse_args()
@@ -204,6 +307,9 @@ def main():
 
     if args.workload == "halo":
         halo_bench(args)
+        return
+    if args.workload == "domain":
+        domain_bench(args)
         return
 
     import hipcstone as hip

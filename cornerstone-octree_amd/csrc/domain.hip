@@ -331,6 +331,37 @@ __global__ void zeroI32Kernel(int32_t* __restrict__ dst, int n)
     if (int(threadIdx.x) < n) dst[threadIdx.x] = 0;
 }
 
+// ---------------------------------------------------------------- property gather (the sync's SFC order)
+
+template<int E>
+struct PropRecord
+{
+    uint8_t b[E];
+};
+template<>
+struct alignas(4) PropRecord<4>
+{
+    uint32_t v;
+};
+template<>
+struct alignas(8) PropRecord<8>
+{
+    uint64_t v;
+};
+template<>
+struct alignas(16) PropRecord<16>
+{
+    uint64_t v[2];
+};
+
+template<int E>
+__global__ void gatherPropKernel(const uint32_t* __restrict__ perm, size_t n, const PropRecord<E>* __restrict__ src,
+                                 PropRecord<E>* __restrict__ dst)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
 // ---------------------------------------------------------------- the domain object
 
 template<class K, class T>
@@ -400,6 +431,8 @@ struct DomainBase
     virtual hipcstone_status sync(hipcstone_stream s, void* x, void* y, void* z, void* h, size_t n,
                                   void* keysOut) = 0;
     virtual void view(hipcstone_domain_view* v) const = 0;
+    virtual hipcstone_status gatherProperties(hipcstone_stream s, void* const* arrays, const size_t* elemBytes,
+                                              int numArrays) = 0;
 };
 
 template<class K, class T>
@@ -503,6 +536,37 @@ public:
         v->centers          = centers_.ptr;
         v->sizes            = sizes_.ptr;
         v->layout           = layout_.ptr;
+    }
+
+    // reorder property arrays (n elements each, in the order the last sync received x, y, z, h) into the SFC order
+    // of that sync, in place through one scratch buffer
+    hipcstone_status gatherProperties(hipcstone_stream s, void* const* arrays, const size_t* elemBytes,
+                                      int numArrays) override
+    {
+        if (n_ == 0) return invalid("domain gather: no sync yet");
+        if (numArrays < 0 || (numArrays > 0 && (!arrays || !elemBytes))) return invalid("domain gather: null list");
+        for (int a = 0; a < numArrays; ++a)
+        {
+            size_t e = elemBytes[a];
+            if (!arrays[a]) return invalid("domain gather: null array");
+            if (e != 1 && e != 2 && e != 4 && e != 8 && e != 12 && e != 16)
+                return invalid("domain gather: element size must be 1, 2, 4, 8, 12 or 16 bytes");
+            if (auto r = propScratch_.reserve(n_ * e)) return r;
+            unsigned blocks = ceilDiv(n_, 256);
+            const uint32_t* perm = perm_.as<uint32_t>();
+            switch (e)
+            {
+                case 1: gatherPropKernel<1><<<blocks, 256, 0, stream(s)>>>(perm, n_, static_cast<const PropRecord<1>*>(arrays[a]), propScratch_.as<PropRecord<1>>()); break;
+                case 2: gatherPropKernel<2><<<blocks, 256, 0, stream(s)>>>(perm, n_, static_cast<const PropRecord<2>*>(arrays[a]), propScratch_.as<PropRecord<2>>()); break;
+                case 4: gatherPropKernel<4><<<blocks, 256, 0, stream(s)>>>(perm, n_, static_cast<const PropRecord<4>*>(arrays[a]), propScratch_.as<PropRecord<4>>()); break;
+                case 8: gatherPropKernel<8><<<blocks, 256, 0, stream(s)>>>(perm, n_, static_cast<const PropRecord<8>*>(arrays[a]), propScratch_.as<PropRecord<8>>()); break;
+                case 12: gatherPropKernel<12><<<blocks, 256, 0, stream(s)>>>(perm, n_, static_cast<const PropRecord<12>*>(arrays[a]), propScratch_.as<PropRecord<12>>()); break;
+                default: gatherPropKernel<16><<<blocks, 256, 0, stream(s)>>>(perm, n_, static_cast<const PropRecord<16>*>(arrays[a]), propScratch_.as<PropRecord<16>>()); break;
+            }
+            HC_CHECK_LAUNCH("gatherPropKernel");
+            HC_CHECK(hipMemcpyAsync(arrays[a], propScratch_.ptr, n_ * e, hipMemcpyDeviceToDevice, stream(s)));
+        }
+        return HIPCSTONE_OK;
     }
 
 private:
@@ -784,7 +848,7 @@ private:
             HC_CHECK(hipMemcpyAsync(fLeavesNew_.ptr, injKeyBuf_.ptr, size_t(numKeys) * sizeof(K),
                                     hipMemcpyDeviceToDevice, stream(s)));
         }
-        std::swap(fLeaves_, fLeavesNew_);
+        swap(fLeaves_, fLeavesNew_);
         fL_ = numKeys - 1;
         if (auto e = linkFocus(s)) return e;
         *converged = conv;
@@ -847,7 +911,7 @@ private:
     DeviceBuf fLeaves_, fLeavesNew_, fLeafCounts_, fNodeCounts_, leafOps_, ops_, prot_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_;
-    DeviceBuf injKeyBuf_, injFlags_, injSortWs_, injScanWs_;
+    DeviceBuf injKeyBuf_, injFlags_, injSortWs_, injScanWs_, propScratch_;
 };
 
 } // namespace hipcstone
@@ -909,6 +973,13 @@ hipcstone_status hipcstone_domain_get_view(const hipcstone_domain* d, hipcstone_
     if (!d || !v) return invalid("domain_get_view: null pointer");
     d->impl->view(v);
     return HIPCSTONE_OK;
+}
+
+hipcstone_status hipcstone_domain_gather_properties(hipcstone_domain* d, hipcstone_stream s, void* const* arrays,
+                                                    const size_t* elem_bytes, int num_arrays)
+{
+    if (!d) return invalid("domain_gather_properties: null handle");
+    return d->impl->gatherProperties(s, arrays, elem_bytes, num_arrays);
 }
 
 hipcstone_status hipcstone_domain_set_timing(hipcstone_domain* d, int enable)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "runtime.hpp"
@@ -9,13 +10,22 @@
 namespace hipcstone
 {
 
+// owns one device allocation; not copyable (a copy would free the allocation twice), swappable
 struct DeviceBuf
 {
     void* ptr    = nullptr;
     size_t bytes = 0;
+    DeviceBuf()  = default;
+    DeviceBuf(const DeviceBuf&)            = delete;
+    DeviceBuf& operator=(const DeviceBuf&) = delete;
     ~DeviceBuf()
     {
         if (ptr) (void)hipFree(ptr);
+    }
+    friend void swap(DeviceBuf& a, DeviceBuf& b) noexcept
+    {
+        std::swap(a.ptr, b.ptr);
+        std::swap(a.bytes, b.bytes);
     }
     hipcstone_status reserve(size_t b)
     {

@@ -53,6 +53,10 @@ constexpr SlotTables makeSlotTables()
 
 __constant__ SlotTables kSlotTables = makeSlotTables();
 
+// dword 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family targets: 32-bit data format, range checked
+// against the byte count in dword 2
+constexpr int kRawBufferFlags = 0x00020000;
+
 template<class T>
 struct alignas(4 * sizeof(T)) Particle4
 {
@@ -62,10 +66,11 @@ struct alignas(4 * sizeof(T)) Particle4
 template<class T>
 struct alignas(4 * sizeof(T)) TargetRec
 {
-    Particle4<T> p; // x, y, z, 4h^2
-    uint32_t count; // neighbors found so far
-    uint32_t pbc;   // periodic-image flags
-    uint64_t rowOff; // element offset of the output row
+    Particle4<T> p;  // x, y, z, 4h^2
+    uint32_t count;  // neighbors found so far
+    uint32_t meta;   // periodic-image flags (bits 0-2) | lane of the target (bits 8-13)
+    uint32_t rowOff; // element offset of the output row from the wave's first row
+    uint32_t pad;
 };
 
 template<class T>
@@ -76,8 +81,8 @@ struct NbShared
     int32_t next[kNbWaves][kMaxDepth];
     int32_t listLeaf[kNbWaves][kLeafList];
     uint64_t listMask[kNbWaves][kLeafList];
-    TargetRec<T> target[kNbWaves][kWave]; // per lane: x, y, z, 4h^2, neighbors so far, pbc flags, row offset
-    uint8_t active[kNbWaves][2 * kWave];  // compacted active target lanes of the current leaf (+ zero padding)
+    TargetRec<T> ctgt[kNbWaves][kWave]; // the current leaf's active targets, compacted in lane order
+    uint32_t dummy[kNbWaves][kWave];    // sink of the count writes of lanes that own no count
 };
 
 template<class T, bool AnyPbc>
@@ -136,19 +141,27 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         return d0 * d0 + (d1 * d1 + d2 * d2);
     };
 
-    // per-target state in LDS, read by whichever lanes test that target
-    sh.target[w][lane] = TargetRec<T>{Particle4<T>{xi, yi, zi, radiusSq}, 0u,
-                                      uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)),
-                                      uint64_t(rowBase + lane) * ngmax};
-    sh.active[w][lane]         = 0;
-    sh.active[w][lane + kWave] = 0;
-    waveSync();
+    // per-target state: the lane of target t keeps its running count in a register; while a leaf is drained the
+    // leaf's active targets are copied, compacted, into LDS for the lanes that test them
+    const TargetRec<T> myRec{Particle4<T>{xi, yi, zi, radiusSq}, 0u,
+                             uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)) | (uint32_t(lane) << 8),
+                             uint32_t(lane) * ngmax, 0u};
+    uint32_t myCount = 0;
+
+    // the wave's 64 output rows as a raw buffer: stores at an offset past its end are dropped by the hardware's
+    // range check, so a lane without a hit stores nowhere instead of branching around the store
+    const uint32_t validRows = uint64_t(lastId) - first < uint64_t(kWave) ? uint32_t(uint64_t(lastId) - first) : kWave;
+    const uint32_t rowsBytes = ngmax ? validRows * ngmax * uint32_t(sizeof(uint32_t)) : 0u;
+    const __amdgpu_buffer_rsrc_t rows =
+        __builtin_amdgcn_make_buffer_rsrc(ngmax ? static_cast<void*>(neighbors + rowBase * ngmax) : nullptr, 0,
+                                          int(rowsBytes), kRawBufferFlags);
 
     // ---- leaf phase. A leaf chunk of nj <= 64 particles is tested against its active targets in pair slots:
-    //      lane = (slot, jj) with slot = lane / nj, jj = lane % nj, S = 64 / nj slots; slot s of iteration it tests
-    //      active target k = it*S + s against particle jj. Within a slot, lanes are ascending jj, so the ballot of
-    //      hits ranks each hit among its target's hits in ascending j and all of a target's hits of the chunk land
-    //      contiguously in its row (coalesced runs). The slot's last lane advances the target's count. ----
+    //      lane = (slot, jj) with slot = lane / nj, jj = lane % nj, S = 64 / nj slots; slot s of an iteration tests
+    //      active targets k = k0 + s and k0 + S + s (two independent targets per lane, for instruction-level
+    //      parallelism) against particle jj. Within a slot, lanes are ascending jj, so the ballot of hits ranks each
+    //      hit among its target's hits in ascending j and all of a target's hits of the chunk land contiguously in
+    //      its row (coalesced runs). The slot's last lane advances the target's count. ----
     int listCount = 0;
     auto drain = [&]() {
         for (int e = 0; e < listCount; ++e)
@@ -158,7 +171,14 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
             uint32_t jb     = uniform(layout[leafIdx]);
             uint32_t je     = uniform(layout[leafIdx + 1]);
             int pc          = __builtin_popcountll(m);
-            if ((m >> lane) & 1u) sh.active[w][laneRank(m)] = uint8_t(lane);
+            const bool mine = (m >> lane) & 1u;
+            const unsigned myRank = laneRank(m);
+            if (mine)
+            {
+                TargetRec<T> r = myRec;
+                r.count        = myCount;
+                sh.ctgt[w][myRank] = r;
+            }
             waveSync();
             for (uint32_t cs = jb; cs < je; cs += kWave)
             {
@@ -182,35 +202,52 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 const bool selfChunk = cs < uint32_t(first) + uint32_t(kWave) && cs + nj > uint32_t(first);
                 const uint32_t jRel  = j - uint32_t(first);
 
-                auto iterate = [&](auto checkSelf) {
-                    for (int k0 = 0; k0 < pc; k0 += int(S))
+                auto test = [&](const TargetRec<T>& rec, bool tv, auto checkSelf) -> bool {
+                    bool fx = false, fy = false, fz = false;
+                    if constexpr (AnyPbc)
                     {
-                        int k              = k0 + int(slot); // < 128: pc <= 64 and slot < S <= 64
-                        bool tv            = jv & (k < pc);
-                        int t              = sh.active[w][k];
-                        const TargetRec<T> rec = sh.target[w][t];
-                        bool fx = false, fy = false, fz = false;
-                        if constexpr (AnyPbc)
-                        {
-                            fx = rec.pbc & 1;
-                            fy = rec.pbc & 2;
-                            fz = rec.pbc & 4;
-                        }
-                        T xx     = fold(xj - rec.p.x, fx, 0);
-                        T yy     = fold(yj - rec.p.y, fy, 1);
-                        T zz     = fold(zj - rec.p.z, fz, 2);
-                        bool hit = tv & (xx * xx + yy * yy + zz * zz < rec.p.w);
-                        if constexpr (decltype(checkSelf)::value) { hit = hit & (jRel != uint32_t(t)); }
-                        uint64_t H    = ballot(hit);
-                        uint32_t pos  = rec.count + laneRank(H & ~below);
-                        if (hit & (pos < ngmax)) neighbors[rec.rowOff + pos] = j;
-                        if (last & tv) sh.target[w][t].count = pos + (hit ? 1u : 0u);
+                        fx = rec.meta & 1;
+                        fy = rec.meta & 2;
+                        fz = rec.meta & 4;
+                    }
+                    T xx     = fold(xj - rec.p.x, fx, 0);
+                    T yy     = fold(yj - rec.p.y, fy, 1);
+                    T zz     = fold(zj - rec.p.z, fz, 2);
+                    bool hit = tv & (xx * xx + yy * yy + zz * zz < rec.p.w);
+                    if constexpr (decltype(checkSelf)::value) { hit = hit & (jRel != (rec.meta >> 8)); }
+                    return hit;
+                };
+                auto iterate = [&](auto checkSelf) {
+                    for (int k0 = 0; k0 < pc; k0 += 2 * int(S))
+                    {
+                        int kA   = k0 + int(slot); // < 128: pc <= 64 and slot < S <= 64
+                        int kB   = kA + int(S);
+                        bool tvA = jv & (kA < pc);
+                        bool tvB = jv & (kB < pc);
+                        const TargetRec<T> recA = sh.ctgt[w][kA < kWave ? kA : kWave - 1];
+                        const TargetRec<T> recB = sh.ctgt[w][kB < kWave ? kB : kWave - 1];
+                        bool hitA     = test(recA, tvA, checkSelf);
+                        bool hitB     = test(recB, tvB, checkSelf);
+                        uint64_t HA   = ballot(hitA);
+                        uint64_t HB   = ballot(hitB);
+                        uint32_t posA = recA.count + laneRank(HA & ~below);
+                        uint32_t posB = recB.count + laneRank(HB & ~below);
+                        uint32_t offA = (hitA & (posA < ngmax)) ? (recA.rowOff + posA) * 4u : rowsBytes;
+                        uint32_t offB = (hitB & (posB < ngmax)) ? (recB.rowOff + posB) * 4u : rowsBytes;
+                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offA), 0, 0);
+                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offB), 0, 0);
+                        uint32_t* cA = (last & tvA) ? &sh.ctgt[w][kA & (kWave - 1)].count : &sh.dummy[w][lane];
+                        uint32_t* cB = (last & tvB) ? &sh.ctgt[w][kB & (kWave - 1)].count : &sh.dummy[w][lane];
+                        *cA          = posA + (hitA ? 1u : 0u);
+                        *cB          = posB + (hitB ? 1u : 0u);
                     }
                 };
                 if (selfChunk) iterate(std::true_type{});
                 else iterate(std::false_type{});
                 waveSync();
             }
+            if (mine) myCount = sh.ctgt[w][myRank].count;
+            waveSync();
         }
         listCount = 0;
     };
@@ -302,8 +339,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         drain();
     }
 
-    waveSync();
-    if (valid) neighborsCount[i - firstId] = sh.target[w][lane].count;
+    if (valid) neighborsCount[i - firstId] = myCount;
 }
 
 template<class T>
@@ -319,6 +355,7 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
         !sizes || !neighborsCount)
         return invalid("findNeighbors: null pointer");
     if (ngmax > 0 && !neighbors) return invalid("findNeighbors: neighbors is null with ngmax > 0");
+    if (ngmax > (1u << 20)) return invalid("findNeighbors: ngmax above 2^20");
     if (numNodes <= 0) return invalid("findNeighbors: numNodes must be positive");
     size_t groups   = (size_t(lastId - firstId) + kWave - 1) / kWave;
     size_t threads  = groups * kWave;

@@ -644,6 +644,14 @@ class Domain:
         _check(_lib.hipcstone_domain_get_view(self.handle, C.byref(v)), "domain_get_view")
         return v
 
+    def gather_properties(self, arrays, stream=None):
+        """Reorder device arrays (in the order the last sync received x, y, z, h) into that sync's SFC order."""
+        arrays = list(arrays)
+        ptrs = (C.c_void_p * max(1, len(arrays)))(*[a.ptr for a in arrays])
+        sizes = (C.c_size_t * max(1, len(arrays)))(*[np.dtype(a.dtype).itemsize for a in arrays])
+        _check(_lib.hipcstone_domain_gather_properties(self.handle, _stream(stream), ptrs, sizes, C.c_int(len(arrays))),
+               "domain_gather_properties")
+
     def set_timing(self, enable=True):
         _check(_lib.hipcstone_domain_set_timing(self.handle, C.c_int(int(enable))), "domain_set_timing")
 

@@ -485,6 +485,12 @@ hipcstone_status hipcstone_domain_destroy(hipcstone_domain* d);
 hipcstone_status hipcstone_domain_sync(hipcstone_domain* d, hipcstone_stream s, void* x, void* y, void* z, void* h,
                                        size_t n, void* keys_out);
 hipcstone_status hipcstone_domain_get_view(const hipcstone_domain* d, hipcstone_domain_view* v);
+/* Particle properties in the order of the last sync. Replaces the particleProperties tuple of Domain::sync
+ * (domain.hpp:169-218, reordered with x,y,z,h by updateLayout :479-537) and reapplySync (domain.hpp:295-324): each of
+ * the num_arrays device arrays (n elements of elem_bytes[a] in {1,2,4,8,12,16}, in the order the last sync received
+ * x, y, z, h) is reordered in place into that sync's SFC order. ---- */
+hipcstone_status hipcstone_domain_gather_properties(hipcstone_domain* d, hipcstone_stream s, void* const* arrays,
+                                                    const size_t* elem_bytes, int num_arrays);
 /* per-stage timing as for the pipeline; stages: 0 box, 1 keys, 2 sort+gather, 3 global tree, 4 focus update,
  * 5 focus counts, 6 centers + layout + copy-out */
 #define HIPCSTONE_DOMAIN_STAGES 7

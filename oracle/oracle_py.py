@@ -447,6 +447,18 @@ class CpuLib:
                 return t[: nn.value + 1].copy(), cc[: nn.value].copy(), steps
             cap = nn.value
 
+    def halo_chain(self, x, y, z, h, box, bucket, num_ranks, rank, curve=MORTON, search_ext=1.0):
+        """The rank-share chain through the reference's CPU functions (ref only): (local particles, stage ms[8]).
+        x, y, z, h are sorted in place."""
+        assert self.kind == "ref"
+        T = x.dtype
+        times = np.zeros(8, dtype=np.float64)
+        f = self._f(f"halo_chain_u64_{_RS[T]}")
+        f.restype = C.c_size_t
+        m = f(C.c_int(curve), _ptr(x), _ptr(y), _ptr(z), _ptr(h), C.c_size_t(x.size), C.byref(box), C.c_uint(bucket),
+              C.c_int(num_ranks), C.c_int(rank), C.c_float(search_ext), _ptr(times))
+        return int(m), times
+
     def omp_threads(self):
         assert self.kind == "ref"
         return int(self.lib.ref_omp_threads())

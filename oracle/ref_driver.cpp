@@ -259,6 +259,85 @@ int pipelineImpl(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>* box
     return nn;
 }
 
+/*
+ * The rank-share chain through the reference's CPU functions, timed per stage: keys, stable sort + gather,
+ * computeOctree, updateInternalTree, nodeFpCenters + layout, makeSfcAssignment, discoverHalos' CPU branch
+ * (octree_focus_mpi.hpp:553-566: search boxes seeded with leaf centers, computeBoundingBox, findHalos),
+ * computeNodeLayout, and the halo gather as extractMarkedElements ranges copied into local arrays.
+ * times[0..7]: keys, sort+gather, tree, link, centers+layout, assignment, halos, layout+gather. Returns the number of
+ * local particles.
+ */
+template<class K, class T>
+size_t haloChainImpl(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>* boxc, unsigned bucket, int numRanks,
+                     int rank, float searchExt, double* times)
+{
+    Box<T> box = toBox(boxc);
+    std::vector<K> keys(n);
+    double t0 = nowMs();
+    if (curve == 1) computeSfcKeys(x, y, z, (HilbertKey<K>*)keys.data(), n, box);
+    else computeSfcKeys(x, y, z, (MortonKey<K>*)keys.data(), n, box);
+    double t1 = nowMs();
+    std::vector<LocalIndex> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    sort_by_key(keys.begin(), keys.end(), perm.begin());
+    std::vector<T> tmp(n);
+    for (T* a : {x, y, z, h})
+    {
+        gather<LocalIndex>(std::span<const LocalIndex>(perm.data(), n), a, tmp.data());
+        std::copy(tmp.begin(), tmp.end(), a);
+    }
+    double t2 = nowMs();
+    auto [tree, counts] = computeOctree<K>(std::span<const K>(keys.data(), n), bucket);
+    double t3 = nowMs();
+    OctreeData<K, execution::Cpu> octree;
+    octree.resize(nNodes(tree));
+    updateInternalTree<K>(tree, octree.data());
+    double t4 = nowMs();
+    TreeNodeIndex L = nNodes(tree);
+    std::vector<LocalIndex> layout(L + 1);
+    layout[0] = 0;
+    std::inclusive_scan(counts.begin(), counts.end(), layout.begin() + 1);
+    std::vector<Vec3<T>> centers(octree.numNodes), sizes(octree.numNodes);
+    nodeFpCenters<K>(std::span<const K>(octree.prefixes.data(), octree.numNodes), centers.data(), sizes.data(), box);
+    double t5 = nowMs();
+    auto assignment = makeSfcAssignment(numRanks, counts, tree.data());
+    TreeNodeIndex first = assignment.treeOffsetsConst()[rank], last = assignment.treeOffsetsConst()[rank + 1];
+    double t6 = nowMs();
+    std::vector<Vec3<T>> sc(L), ss(L);
+    auto l2i = leafToInternal(octree);
+    gather<TreeNodeIndex>(l2i, centers.data(), sc.data());
+#pragma omp parallel for schedule(static)
+    for (TreeNodeIndex i = first; i < last; ++i)
+        std::tie(sc[i], ss[i]) = computeBoundingBox(x, y, z, h, layout[i], layout[i + 1], T(2 * searchExt), sc[i]);
+    std::vector<uint8_t> flags(octree.numNodes, 0);
+    findHalos(octree.prefixes.data(), octree.childOffsets.data(), octree.parents.data(), centers.data(),
+              sizes.data(), tree.data(), sc.data(), ss.data(), box, first, last, flags.data());
+    double t7 = nowMs();
+    std::vector<LocalIndex> local(L + 1);
+    computeNodeLayout(execution::Cpu{}, std::span<const unsigned>(counts.data(), L),
+                      std::span<const uint8_t>(flags.data(), flags.size()), l2i, TreeIndexPair(first, last),
+                      std::span<LocalIndex>(local.data(), local.size()));
+    auto ranges = extractMarkedElements<LocalIndex>(std::span<const LocalIndex>(layout.data(), layout.size()),
+                                                   std::span<const LocalIndex>(local.data(), local.size()), 0, L);
+    size_t numLocal = local[L];
+    std::vector<size_t> dstOff(ranges.size() / 2 + 1, 0);
+    for (size_t r = 0; r + 1 < ranges.size(); r += 2)
+        dstOff[r / 2 + 1] = dstOff[r / 2] + (ranges[r + 1] - ranges[r]);
+    std::vector<T> outs[4];
+    T* srcs[4] = {x, y, z, h};
+    for (int a = 0; a < 4; ++a)
+    {
+        outs[a].resize(numLocal);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (size_t r = 0; r < ranges.size() / 2; ++r)
+            std::copy(srcs[a] + ranges[2 * r], srcs[a] + ranges[2 * r + 1], outs[a].data() + dstOff[r]);
+    }
+    double t8 = nowMs();
+    double tt[8] = {t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4, t6 - t5, t7 - t6, t8 - t7};
+    std::copy(tt, tt + 8, times);
+    return numLocal;
+}
+
 } // namespace
 
 extern "C"
@@ -450,6 +529,11 @@ extern "C"
     {                                                                                                                  \
         findHalosImpl<K, T>(prefixes, childOffsets, parents, nodeCenters, nodeSizes, leaves, searchCenters,           \
                             searchSizes, box, firstNode, lastNode, flags);                                             \
+    }                                                                                                                  \
+    size_t ref_halo_chain_##KS##_##TS(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>* box,             \
+                                      unsigned bucket, int numRanks, int rank, float searchExt, double* times)          \
+    {                                                                                                                  \
+        return haloChainImpl<K, T>(curve, x, y, z, h, n, box, bucket, numRanks, rank, searchExt, times);              \
     }                                                                                                                  \
     int ref_pipeline_##KS##_##TS(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>* box, unsigned bucket,   \
                                  K* tree, int* numNodes, unsigned* counts, int cap, unsigned ngmax,                    \

@@ -104,3 +104,23 @@ def test_domain_rejects_bad_arguments(hip):
     x = hip.DeviceArray(10, np.float32)
     with pytest.raises(hip.HipcstoneError):
         dom.sync(x, x, x, None, 10)
+
+
+def test_domain_gathers_properties_in_sync_order(hip, oracle):
+    """Properties follow the sync's SFC order (Domain::sync particleProperties / reapplySync at one rank)."""
+    n, T, kd = 50000, np.float64, np.uint64
+    x, y, z, h = plummer_input(n, T, seed=34)
+    dom = hip.Domain(hip.make_box(-1.0, 1.0, T), key_bits=64, bucket_size=32, bucket_size_focus=8)
+    ref = CpuDomain(oracle, kd, T, 32, 8, -1.0, 1.0)
+    d = [dev(hip, a) for a in (x, y, z, h)]
+    ids = np.arange(n, dtype=np.uint32)
+    props = [dev(hip, ids), dev(hip, (ids * 3).astype(np.uint64)), dev(hip, ids.astype(np.uint8)),
+             dev(hip, ids.astype(np.uint16))]
+    o = ref.sync(x, y, z, h)
+    dom.sync(*d, n)
+    dom.gather_properties(props)
+    perm = o["perm"]
+    assert np.array_equal(props[0].download(), ids[perm])
+    assert np.array_equal(props[1].download(), (ids * 3).astype(np.uint64)[perm])
+    assert np.array_equal(props[2].download(), ids.astype(np.uint8)[perm])
+    assert np.array_equal(props[3].download(), ids.astype(np.uint16)[perm])

@@ -228,3 +228,27 @@ def test_discover_halos_vs_ref_parts(oracle, ref, kd, T):
     scan = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint32)
     xs2 = oracle.gather_ranges(scan, pairs[:, 0], p["x"], int(lens.sum()))
     assert np.array_equal(xs, xs2)
+
+
+def test_reference_halo_chain_matches_oracle_chain(oracle, ref):
+    """The reference's CPU rank-share chain (used as the halo benchmark's CPU baseline) and the oracle's chain agree
+    on the number of local particles."""
+    from hipcstone import inputs
+
+    T = np.float64
+    n = 200000
+    x, y, z = inputs.gaussian_mixture(n, clusters=16, seed=14, dtype=T)
+    h = np.full(n, 0.004, dtype=T)
+    box = make_box(0, 1, T)
+    p = cpu_pipeline(oracle, x, y, z, h, box, np.uint64, 128, MORTON)
+    off, _, _ = oracle.make_sfc_assignment(p["leaves"], p["counts"], 8)
+    first, last = int(off[2]), int(off[3])
+    t = {k: p[k] for k in ("prefixes", "childOffsets", "parents", "leafToInternal")}
+    _, _, flags = oracle.discover_halos(t, p["centers"], p["sizes"], p["leaves"], p["x"], p["y"], p["z"], p["h"],
+                                        p["layout"], first, last, box)
+    L = p["counts"].size
+    local = oracle.compute_node_layout(p["counts"], flags, p["leafToInternal"][p["prefixes"].size - L:], first, last)
+    xs, ys, zs, hs = (a.copy() for a in (x, y, z, h))
+    m, times = ref.halo_chain(xs, ys, zs, hs, box, 128, 8, 2)
+    assert m == int(local[-1]) and m > int(p["layout"][last] - p["layout"][first])
+    assert np.array_equal(xs, p["x"]) and times.sum() > 0
