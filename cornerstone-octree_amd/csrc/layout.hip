@@ -403,7 +403,7 @@ HIPCSTONE_DEF_ASSIGN(u64, uint64_t)
         const int32_t* leaf_to_internal, const T* node_centers, const T* node_sizes, const K* leaves,                   \
         int32_t num_nodes, const T* x, const T* y, const T* z, const T* h, const uint32_t* layout,                     \
         int32_t first_node, int32_t last_node, const hipcstone_box_##RS* box, float search_ext, T* search_centers,     \
-        T* search_sizes, uint8_t* flags, int32_t accumulate)                                                           \
+        T* search_sizes, uint8_t* flags, int32_t accumulate, void* workspace, size_t workspace_bytes)                  \
     {                                                                                                                  \
         if (!leaf_to_internal || !node_centers || !search_centers || !search_sizes || !flags || !box)                 \
             return invalid("discoverHalos: null pointer");                                                            \
@@ -426,7 +426,7 @@ HIPCSTONE_DEF_ASSIGN(u64, uint64_t)
         }                                                                                                              \
         return hipcstone_find_halos_##KS##_##RS(s, prefixes, child_offsets, parents, node_centers, node_sizes,        \
                                                 leaves, search_centers, search_sizes, box, first_node, last_node,      \
-                                                num_nodes, flags);                                                     \
+                                                num_nodes, flags, workspace, workspace_bytes);                         \
     }
 HIPCSTONE_DEF_DISCOVER(u32, uint32_t, f32, float)
 HIPCSTONE_DEF_DISCOVER(u32, uint32_t, f64, double)

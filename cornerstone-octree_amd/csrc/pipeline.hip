@@ -86,10 +86,10 @@ struct Api;
                                          const int32_t* l2i, const T* c, const T* sz, const K* leaves, int32_t N,      \
                                          const T* x, const T* y, const T* z, const T* h, const uint32_t* lay,          \
                                          int32_t first, int32_t last, const Box* b, float ext, T* sc, T* ss,           \
-                                         uint8_t* flags)                                                               \
+                                         uint8_t* flags, void* ws, size_t wsb)                                         \
         {                                                                                                              \
             return hipcstone_discover_halos_##KS##_##RS(s, pf, co, par, l2i, c, sz, leaves, N, x, y, z, h, lay, first, \
-                                                        last, b, ext, sc, ss, flags, 0);                               \
+                                                        last, b, ext, sc, ss, flags, 0, ws, wsb);                      \
         }                                                                                                              \
         static hipcstone_status gatherLeaves(hipcstone_stream s, const uint32_t* src, const uint32_t* dst, int32_t L, \
                                              const T* x, const T* y, const T* z, const T* h, T* xo, T* yo, T* zo,      \
@@ -194,6 +194,7 @@ public:
         if (auto e = assignCnt_.reserve(R * sizeof(uint32_t))) return e;
         if (auto e = assignWs_.reserve(hipcstone_sfc_assignment_workspace_size(L))) return e;
         if (auto e = flags_.reserve(size_t(N))) return e;
+        if (auto e = haloWs_.reserve(hipcstone_find_halos_workspace_size(N))) return e;
         if (auto e = searchC_.reserve(size_t(L) * 3 * sizeof(T))) return e;
         if (auto e = searchS_.reserve(size_t(L) * 3 * sizeof(T))) return e;
         if (auto e = localLayout_.reserve((size_t(L) + 1) * sizeof(uint32_t))) return e;
@@ -221,7 +222,7 @@ public:
         HC_STAGE(8, A::discover(s, prefixes_.as<K>(), childOffsets_.as<int32_t>(), parents_.as<int32_t>(),
                                 leafToInternalLeaves, centers_.as<T>(), sizes_.as<T>(), tree_[treeCur_].template as<K>(),
                                 N, x, y, z, h, layout_.as<uint32_t>(), first_, last_, &box_, ext, searchC_.as<T>(),
-                                searchS_.as<T>(), flags_.as<uint8_t>()));
+                                searchS_.as<T>(), flags_.as<uint8_t>(), haloWs_.ptr, haloWs_.bytes));
         HC_STAGE(9, hipcstone_compute_node_layout(s, counts_[treeCur_].template as<uint32_t>(), flags_.as<uint8_t>(),
                                                   leafToInternalLeaves, L, first_, last_, localLayout_.as<uint32_t>(),
                                                   scanWs_.ptr, scanWs_.bytes));
@@ -424,7 +425,7 @@ private:
     DeviceBuf tree_[2], counts_[2], updWs_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_;
-    DeviceBuf assignOff_, assignBnd_, assignCnt_, assignWs_, flags_, searchC_, searchS_, localLayout_;
+    DeviceBuf assignOff_, assignBnd_, assignCnt_, assignWs_, flags_, searchC_, searchS_, localLayout_, haloWs_;
 };
 
 } // namespace hipcstone

@@ -381,12 +381,20 @@ def compute_bounding_box(x, y, z, h, layout, first, last, scale, centers, sizes,
              ctype(scale), _p(centers), _p(sizes)), "compute_bounding_box")
 
 
+def find_halos_workspace_size(num_nodes):
+    _lib.hipcstone_find_halos_workspace_size.restype = C.c_size_t
+    return int(_lib.hipcstone_find_halos_workspace_size(C.c_int32(num_nodes)))
+
+
 def find_halos(prefixes, child_offsets, parents, centers, sizes, leaves, search_centers, search_sizes, box, first,
-               last, num_nodes, flags, stream=None):
+               last, num_nodes, flags, workspace=None, workspace_bytes=None, stream=None):
+    """findHalos (collisions.hpp:68-94). workspace: a DeviceArray (its size, or workspace_bytes of it, bounds the
+    walk's frame queues) or None for a stream-ordered allocation inside the call."""
     f = _fn(f"hipcstone_find_halos_{_KS[prefixes.dtype]}_{_RS[centers.dtype]}")
+    wsb = 0 if workspace is None else (workspace.nbytes if workspace_bytes is None else int(workspace_bytes))
     _check(f(_stream(stream), _p(prefixes), _p(child_offsets), _p(parents), _p(centers), _p(sizes), _p(leaves),
              _p(search_centers), _p(search_sizes), C.byref(box), C.c_int32(first), C.c_int32(last),
-             C.c_int32(num_nodes), _p(flags)), "find_halos")
+             C.c_int32(num_nodes), _p(flags), _p(workspace), C.c_size_t(wsb)), "find_halos")
 
 
 def _dtype_of(a, default=None):
@@ -414,7 +422,7 @@ def make_sfc_assignment(leaves, counts, num_leaves, num_ranks, key_dtype=None, s
 
 def discover_halos(prefixes, child_offsets, parents, leaf_to_internal, centers, sizes, leaves, num_nodes, x, y, z, h,
                    layout, first, last, box, search_centers, search_sizes, flags, search_ext=1.0, accumulate=False,
-                   key_dtype=None, stream=None):
+                   key_dtype=None, workspace=None, stream=None):
     """FocusedOctree::discoverHalos (octree_focus_mpi.hpp:511-568) on the device; leaf_to_internal is the leaf part
     of the map and layout the absolute leaf layout of x..h."""
     K = _dtype_of(prefixes, key_dtype)
@@ -423,7 +431,8 @@ def discover_halos(prefixes, child_offsets, parents, leaf_to_internal, centers, 
     _check(f(_stream(stream), _p(prefixes), _p(child_offsets), _p(parents), _p(leaf_to_internal), _p(centers),
              _p(sizes), _p(leaves), C.c_int32(num_nodes), _p(x), _p(y), _p(z), _p(h), _p(layout), C.c_int32(first),
              C.c_int32(last), C.byref(box), C.c_float(search_ext), _p(search_centers), _p(search_sizes), _p(flags),
-             C.c_int32(int(accumulate))), "discover_halos")
+             C.c_int32(int(accumulate)), _p(workspace), C.c_size_t(0 if workspace is None else workspace.nbytes)),
+           "discover_halos")
 
 
 def compute_node_layout(counts, flags, leaf_to_internal, num_leaves, first, last, layout, stream=None):

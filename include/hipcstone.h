@@ -252,12 +252,17 @@ HIPCSTONE_DECL_BBOX(f64, double)
 /* ---- a14: halo discovery. Replaces findHalosGpu(exec, prefixes, childOffsets, parents, nodeCenters, nodeSizes,
  * leaves, searchCenters, searchSizes, box, firstNode, lastNode, collisionFlags), traversal/collisions_gpu.h:46-59
  * (CPU collisions.hpp:68-94). flags (node-indexed) are set to 1 for halo nodes and never cleared. num_nodes is the
- * size of the linked tree (bounds every walk). ---- */
+ * size of the linked tree (bounds every walk). workspace holds the walk's frame queues: any size of at least
+ * 256 + 32 * max(64, ceil((last_node - first_node) / 64)) bytes gives the same flags (frames that do not fit are
+ * walked depth first by one wave); hipcstone_find_halos_workspace_size(num_nodes) is large enough to balance the
+ * whole walk over the GPU. A NULL workspace is allocated and freed on the stream (hipMallocAsync). ---- */
+size_t hipcstone_find_halos_workspace_size(int32_t num_nodes);
 #define HIPCSTONE_DECL_HALOS(KS, K, RS, T)                                                                             \
     hipcstone_status hipcstone_find_halos_##KS##_##RS(                                                                 \
         hipcstone_stream s, const K* prefixes, const int32_t* child_offsets, const int32_t* parents,                  \
         const T* node_centers, const T* node_sizes, const K* leaves, const T* search_centers, const T* search_sizes,   \
-        const hipcstone_box_##RS* box, int32_t first_node, int32_t last_node, int32_t num_nodes, uint8_t* flags);
+        const hipcstone_box_##RS* box, int32_t first_node, int32_t last_node, int32_t num_nodes, uint8_t* flags,       \
+        void* workspace, size_t workspace_bytes);
 HIPCSTONE_DECL_HALOS(u32, uint32_t, f32, float)
 HIPCSTONE_DECL_HALOS(u32, uint32_t, f64, double)
 HIPCSTONE_DECL_HALOS(u64, uint64_t, f32, float)
@@ -285,14 +290,15 @@ HIPCSTONE_DECL_ASSIGN(u64, uint64_t)
  * marks every node that overlaps a search box and lies outside [leaves[first_node], leaves[last_node]).
  * Difference: layout holds the absolute offsets of every leaf's particles in x..h (the reference scans the counts of
  * the assigned leaves into a layout relative to the first assigned particle). search_centers / search_sizes are
- * leaf-indexed Vec3 arrays; only [first_node, last_node) is written. ---- */
+ * leaf-indexed Vec3 arrays; only [first_node, last_node) is written. workspace / workspace_bytes as for
+ * hipcstone_find_halos_*. ---- */
 #define HIPCSTONE_DECL_DISCOVER(KS, K, RS, T)                                                                          \
     hipcstone_status hipcstone_discover_halos_##KS##_##RS(                                                             \
         hipcstone_stream s, const K* prefixes, const int32_t* child_offsets, const int32_t* parents,                  \
         const int32_t* leaf_to_internal, const T* node_centers, const T* node_sizes, const K* leaves,                  \
         int32_t num_nodes, const T* x, const T* y, const T* z, const T* h, const uint32_t* layout,                     \
         int32_t first_node, int32_t last_node, const hipcstone_box_##RS* box, float search_ext, T* search_centers,     \
-        T* search_sizes, uint8_t* flags, int32_t accumulate);
+        T* search_sizes, uint8_t* flags, int32_t accumulate, void* workspace, size_t workspace_bytes);
 HIPCSTONE_DECL_DISCOVER(u32, uint32_t, f32, float)
 HIPCSTONE_DECL_DISCOVER(u32, uint32_t, f64, double)
 HIPCSTONE_DECL_DISCOVER(u64, uint64_t, f32, float)

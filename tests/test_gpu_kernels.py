@@ -294,11 +294,21 @@ def test_bbox_and_halos(hip, oracle, kd, T):
     gc = dsc.download().reshape(L, 3)
     gs = dss.download().reshape(L, 3)
     assert np.array_equal(gc[first:last], rc[first:last]) and np.array_equal(gs[first:last], rs[first:last])
-    flags = hip.DeviceArray(p["prefixes"].size, np.uint8)
-    flags.fill(0)
-    hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dev(hip, p["centers"].reshape(-1)),
-                   dev(hip, p["sizes"].reshape(-1)), d["leaves"], dsc, dss, gbox, first, last, p["prefixes"].size,
-                   flags)
     ref = oracle.find_halos(p["prefixes"], p["childOffsets"], p["parents"], p["centers"], p["sizes"], p["leaves"], rc,
                             rs, obox, first, last)
-    assert np.array_equal(flags.download(), ref) and ref.sum() > 0
+    assert ref.sum() > 0
+    num_nodes = p["prefixes"].size
+    groups = (last - first + 63) // 64
+    dc, dsz = dev(hip, p["centers"].reshape(-1)), dev(hip, p["sizes"].reshape(-1))
+    # None: allocated in the call; the smallest workspace sends nearly every frame down the depth-first path; the
+    # recommended one balances everything through the level queues
+    for ws in (None, hip.DeviceArray(256 + 32 * max(64, groups), np.uint8),
+               hip.DeviceArray(hip.find_halos_workspace_size(num_nodes), np.uint8)):
+        flags = hip.DeviceArray(num_nodes, np.uint8)
+        flags.fill(0)
+        hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dc, dsz, d["leaves"], dsc, dss, gbox, first,
+                       last, num_nodes, flags, workspace=ws)
+        assert np.array_equal(flags.download(), ref), None if ws is None else ws.nbytes
+    with pytest.raises(hip.HipcstoneError):
+        hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dc, dsz, d["leaves"], dsc, dss, gbox, first,
+                       last, num_nodes, flags, workspace=hip.DeviceArray(256 + 32 * max(64, groups) - 32, np.uint8))
