@@ -4,6 +4,7 @@
 // (findCollisions, findHalos) with traversal/boxoverlap.hpp:127-168 (containedIn) and :280-291 (overlap).
 // GPU interface: focus/source_center_gpu.h:40-51, traversal/collisions_gpu.h:46-59.
 #include <algorithm>
+#include <limits>
 
 #include "common.hpp"
 #include "runtime.hpp"
@@ -100,303 +101,322 @@ __device__ inline bool boxContainedInRange(K codeStart, K codeEnd, const T c[3],
 
 // findHalos (collisions.hpp:25-94): for each local leaf, a walk from the root that calls, at every node it reaches,
 // overlaps(node) = !containedIn(node keys, [lowestKey, highestKey)) && overlap(node box, search box), flags the node
-// when it returns true and descends into the node's children only then. So leaf l flags node n iff overlaps() holds
-// for l at n and at every ancestor of n; leaves whose search box lies inside the local key range walk nothing.
+// when it returns true and descends into the node's children only then. So node n is flagged iff some active leaf l
+// (a local leaf whose search box is not inside the local key range, :88-89) passes overlaps() at n and at every
+// ancestor of n. A node with an unflagged parent is never flagged.
 //
-// GPU design. The 64 consecutive local leaves of a group share most of their walks, so a wave walks the union of a
-// group's walks: expanding a node tests its 8 children at once (their boxes, prefixes and child offsets are contiguous
-// and wave-uniform) for the lanes that passed the node and keeps the per-child ballot of passing lanes. A child with a
-// non-empty ballot is flagged (one byte store per node and group instead of one per leaf) and, if internal, becomes a
-// frame (group, first child, ballot). Each lane is tested exactly where its own reference walk tests, so the flag set
-// is the reference's.
-//
-// Union walks differ in length by orders of magnitude (a group next to a dense cluster expands tens of thousands of
-// frames, the median group a few hundred), so one wave per group leaves the GPU waiting on a few waves. The frames are
-// instead expanded level by level: level d reads the frames of depth-d nodes from a queue and appends the frames of
-// their children to the next queue, with every wave of a fixed grid taking frames round-robin. Waves stage their
-// output frames in LDS and reserve queue space with one atomic per batch. A frame that finds its queue full is walked
-// to the bottom by the wave that made it (depth first, with an explicit stack in LDS), so any queue capacity gives the
-// same flags; the capacity only decides how much of the work is balanced.
-constexpr int kHaloThreads     = 256;
-constexpr int kHaloWaves       = kHaloThreads / kWave;
-constexpr int kHaloMaxDepth    = 22;
-constexpr int kHaloStage       = 64;   // staged output frames per wave
-constexpr unsigned kHaloBlocks = 2048; // grid of the level kernels (8 waves per SIMD)
-constexpr size_t kHaloCounters = 32;   // uint64 queue counters at the start of the workspace, one per level
+// GPU design: the tree is walked once, top down, over flagged nodes only, instead of once per leaf. Each flagged
+// internal node n carries a witness: an active leaf that passes n and all of n's ancestors. A child c of n is
+//   - unflagged if it lies inside the local key range (containedIn holds for c, so for every leaf);
+//   - flagged with n's witness if that leaf passes c (its chain to n is already known);
+//   - otherwise flagged iff some active leaf passes c and all of c's ancestors: candidates come from a query of an
+//     8-ary hierarchy of conservative bounds over the active search boxes, and each candidate's chain is checked
+//     with the reference's own test up to the root; the first candidate that passes becomes c's witness.
+// Each test is the reference's overlaps() on the reference's operands, so the flag set is the reference's, while the
+// work is a few tests per flagged node instead of one walk per leaf (the union walks of leaf groups next to a dense
+// cluster re-expand the same subtrees tens of thousands of times).
+constexpr int kHaloThreads      = 256;
+constexpr unsigned kHaloBlocks  = 2048; // grid of the level kernels (grid-stride over the frontier)
+constexpr int kBvhMaxLevels     = 12;   // 8^11 > 2^31 local leaves
+constexpr size_t kHaloHeadBytes = 512;  // level starts and counts, root witness
 
-struct HaloFrame
+// Conservative bounds of the active search boxes: level 0 holds one box per local leaf (empty for inactive leaves),
+// level k+1 the union of 8 consecutive level-k boxes, up to a single box at level top.
+struct HaloBvhShape
 {
-    uint64_t mask; // lanes of the group that passed the parent node
-    int32_t group; // leaf group: local leaves firstNode + 64 * group + lane
-    int32_t co;    // first of the 8 children to test
+    int32_t offset[kBvhMaxLevels]; // first box of each level
+    int32_t count[kBvhMaxLevels];
+    int32_t top;
 };
 
-struct HaloShared
+struct HaloCounters
 {
-    HaloFrame stage[kHaloWaves][kHaloStage];
-    uint64_t mask[kHaloWaves][kHaloMaxDepth][8];
-    int32_t cco[kHaloWaves][kHaloMaxDepth][8]; // first child of each of the frame's 8 nodes
-    int32_t co[kHaloWaves][kHaloMaxDepth];
-    int32_t next[kHaloWaves][kHaloMaxDepth];
+    uint32_t start[32]; // frontier entries of depth d are front[start[d], start[d] + count[d])
+    uint32_t count[32];
+    int32_t rootWitness;
+    HaloBvhShape bvh;
 };
+static_assert(sizeof(HaloCounters) <= kHaloHeadBytes, "findHalos counters overlap the frontier");
+
+// Relative slack of the bounds: larger than the rounding error of the reference's overlap() and of the bounds
+// themselves, so a leaf that passes overlap() in floating point always lies in bounds that pass the bound test.
+template<class T>
+__host__ __device__ constexpr T boundsSlack()
+{
+    return sizeof(T) == 4 ? T(1.0 / (1 << 17)) : T(1.0 / 35184372088832.0); // 2^-17, 2^-45
+}
 
 template<class K, class T>
-struct HaloTree
+struct HaloArgs
 {
     const K* prefixes;
     const int32_t* childOffsets;
+    const int32_t* parents;
     const T* nodeCenters;
     const T* nodeSizes;
     const T* searchCenters;
     const T* searchSizes;
+    const K* leaves; // the local key range is [leaves[firstNode], leaves[lastNode])
     typename BoxOf<T>::type box;
     int32_t firstNode, lastNode, numNodes;
     uint8_t* flags;
 };
 
-// Per-lane state of one group: the lane's search box and the local key range.
-template<class K, class T>
-struct HaloLane
+// containedIn(nodeStart, nodeEnd, lowestKey, highestKey) of the node with this prefix (boxoverlap.hpp:163-168)
+template<class K>
+__device__ inline bool excludedPrefix(K pfx, K lowestKey, K highestKey)
 {
-    T tC[3], tS[3];
-    K lowestKey, highestKey;
-    bool pbc[3];
-
-    __device__ void init(const HaloTree<K, T>& t, K lo, K hi)
-    {
-        lowestKey  = lo;
-        highestKey = hi;
-        for (int d = 0; d < 3; ++d)
-            pbc[d] = t.box.bnd[d] == HIPCSTONE_PERIODIC;
-    }
-
-    // the lane's search box for a group; lanes outside the mask keep stale values that are never used
-    __device__ void load(const HaloTree<K, T>& t, int32_t group, uint64_t mask, int lane)
-    {
-        if ((mask >> lane) & 1u)
-        {
-            size_t leaf = size_t(t.firstNode) + size_t(group) * kWave + size_t(lane);
-            for (int d = 0; d < 3; ++d)
-            {
-                tC[d] = t.searchCenters[leaf * 3 + d];
-                tS[d] = t.searchSizes[leaf * 3 + d];
-            }
-        }
-    }
-
-    // containedIn(nodeStart, nodeEnd, lowestKey, highestKey) of the node with this prefix (boxoverlap.hpp:163-168)
-    __device__ bool excluded(K pfx) const
-    {
-        unsigned pl = decodePrefixLength(pfx);
-        K ret       = pfx ^ (K(1) << pl);
-        int ns      = 3 * int(maxLevel<K>()) - int(pl);
-        K nk1       = ret << ns;
-        K nk2       = nk1 + (K(1) << ns);
-        return !(nk1 < lowestKey || nk2 > highestKey);
-    }
-
-    // overlap(node box, search box) with the reference's operation order (boxoverlap.hpp:280-291)
-    __device__ bool overlapBox(const T* c, const T* s, const typename BoxOf<T>::type& box) const
-    {
-        bool ov = true;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-        {
-            T dx = tC[d] - c[d];
-            if (pbc[d]) dx = dx - box.len[d] * rint(dx * box.ilen[d]);
-            dx = fabs(dx);
-            dx -= s[d];
-            dx -= tS[d];
-            ov = ov && dx < T(0);
-        }
-        return ov;
-    }
-};
-
-// Test the 8 children at co for the lanes in pm: mk[k] = ballot of lanes for which child k overlaps, cco[k] = its
-// first child. Lanes 0-7 flag the children some lane passes.
-template<class K, class T>
-__device__ inline void testChildren(const HaloTree<K, T>& t, const HaloLane<K, T>& L, int32_t co, uint64_t pm,
-                                    int lane, uint64_t mk[8], int32_t cco[8])
-{
-    const bool eligible = (pm >> lane) & 1u;
-    const T* c          = t.nodeCenters + size_t(co) * 3;
-    const T* s          = t.nodeSizes + size_t(co) * 3;
-    T cv[24], sv[24];
-#pragma unroll
-    for (int k = 0; k < 24; ++k)
-    {
-        cv[k] = c[k];
-        sv[k] = s[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-        bool ex = L.excluded(t.prefixes[co + k]);
-        cco[k]  = t.childOffsets[co + k];
-        mk[k]   = ballot(eligible && !ex && L.overlapBox(cv + 3 * k, sv + 3 * k, t.box));
-    }
-    if (lane < 8)
-    {
-        uint64_t mine = mk[0];
-#pragma unroll
-        for (int k = 1; k < 8; ++k)
-            mine = lane == k ? mk[k] : mine;
-        if (mine != 0) t.flags[co + lane] = 1;
-    }
+    unsigned pl = decodePrefixLength(pfx);
+    K ret       = pfx ^ (K(1) << pl);
+    int ns      = 3 * int(maxLevel<K>()) - int(pl);
+    K nk1       = ret << ns;
+    K nk2       = nk1 + (K(1) << ns);
+    return !(nk1 < lowestKey || nk2 > highestKey);
 }
 
-// child k of the frame at co is an internal node whose 8 children lie inside the tree (malformed trees stop here)
-__device__ inline bool descendable(int32_t co, int k, int32_t cco, int32_t numNodes)
+// overlap(node box, search box) with the reference's operation order (boxoverlap.hpp:280-291)
+template<class T>
+__device__ inline bool overlapBox(const T c[3], const T s[3], const T tC[3], const T tS[3],
+                                  const typename BoxOf<T>::type& box)
 {
-    return cco > co + k && cco + 8 <= numNodes;
+    bool ov = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        T dx = tC[d] - c[d];
+        if (box.bnd[d] == HIPCSTONE_PERIODIC) dx = dx - box.len[d] * rint(dx * box.ilen[d]);
+        dx = fabs(dx);
+        dx -= s[d];
+        dx -= tS[d];
+        ov = ov && dx < T(0);
+    }
+    return ov;
 }
 
-// Depth-first walk of everything below one frame, for frames that found no room in the level queue.
-template<class K, class T>
-__device__ void walkFrame(const HaloTree<K, T>& t, HaloLane<K, T>& L, HaloShared& sh, int w, int lane,
-                          const HaloFrame f)
+template<class T>
+__device__ inline void load3(const T* p, int64_t i, T v[3])
 {
-    auto push = [&](int depth, int32_t co, uint64_t pm) {
-        uint64_t mk[8];
-        int32_t cco[8];
-        testChildren(t, L, co, pm, lane, mk, cco);
-        if (lane == 0)
-        {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-            {
-                sh.mask[w][depth][k] = descendable(co, k, cco[k], t.numNodes) ? mk[k] : 0;
-                sh.cco[w][depth][k]  = cco[k];
-            }
-            sh.co[w][depth]   = co;
-            sh.next[w][depth] = 0;
-        }
-        waveSync();
-    };
+    v[0] = p[i * 3 + 0];
+    v[1] = p[i * 3 + 1];
+    v[2] = p[i * 3 + 2];
+}
 
-    L.load(t, f.group, f.mask, lane);
-    int depth = 0;
-    push(0, f.co, f.mask);
-    int64_t stepsLeft = 2 * int64_t(t.numNodes) + 8;
-    while (depth >= 0 && stepsLeft-- > 0)
+// leaf l's search box passes node a and every ancestor of a
+template<class K, class T>
+__device__ inline bool chainPasses(const HaloArgs<K, T>& A, int32_t a, const T tC[3], const T tS[3])
+{
+    for (int step = 0; step < 64; ++step)
     {
-        int k      = uniform(sh.next[w][depth]);
-        uint64_t m = 0;
-        while (k < 8)
+        T c[3], s[3];
+        load3(A.nodeCenters, a, c);
+        load3(A.nodeSizes, a, s);
+        if (!overlapBox(c, s, tC, tS, A.box)) return false;
+        if (a == 0) return true;
+        a = A.parents[(a - 1) / 8];
+        if (a < 0 || a >= A.numNodes) return false; // malformed tree
+    }
+    return false;
+}
+
+// bound test of node box (c, s) against stored bounds; for periodic dimensions the minimum image of the bounds' center
+template<class T>
+__device__ inline bool boundsOverlap(const T c[3], const T s[3], const T* b, const typename BoxOf<T>::type& box)
+{
+    bool ov = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        T lo = b[d], hi = b[3 + d];
+        if (box.bnd[d] == HIPCSTONE_PERIODIC)
         {
-            m = uniform64(sh.mask[w][depth][k]);
-            if (m != 0) break;
+            T m  = (lo + hi) * T(0.5);
+            T r  = (hi - lo) * T(0.5);
+            T dx = c[d] - m;
+            dx   = dx - box.len[d] * rint(dx * box.ilen[d]);
+            ov   = ov && fabs(dx) < s[d] + r;
+        }
+        else { ov = ov && c[d] + s[d] > lo && c[d] - s[d] < hi; }
+    }
+    return ov;
+}
+
+// First active leaf (relative index) that passes node c (box cb, sb) and all of c's ancestors starting at parent n,
+// or -1. Stackless walk of the implicit 8-ary bounds hierarchy.
+template<class K, class T>
+__device__ int32_t findWitness(const HaloArgs<K, T>& A, const T* bvh, const HaloBvhShape& H, int32_t n, const T cb[3],
+                               const T sb[3])
+{
+    int k     = H.top;
+    int32_t i = 0;
+    for (;;)
+    {
+        if (i < H.count[k] && boundsOverlap(cb, sb, bvh + size_t(H.offset[k] + i) * 6, A.box))
+        {
+            if (k > 0)
+            {
+                --k;
+                i *= 8;
+                continue;
+            }
+            T tC[3], tS[3];
+            int64_t leaf = int64_t(A.firstNode) + i;
+            load3(A.searchCenters, leaf, tC);
+            load3(A.searchSizes, leaf, tS);
+            if (overlapBox(cb, sb, tC, tS, A.box) && chainPasses(A, n, tC, tS)) return i;
+        }
+        while (k < H.top && ((i & 7) == 7 || i + 1 >= H.count[k]))
+        {
+            i >>= 3;
             ++k;
         }
-        if (k >= 8)
-        {
-            --depth;
-            continue;
-        }
-        int32_t cco = uniform(sh.cco[w][depth][k]);
-        if (lane == 0) sh.next[w][depth] = k + 1;
-        waveSync();
-        if (depth + 1 >= kHaloMaxDepth) break; // deeper than any key allows: malformed tree
-        ++depth;
-        push(depth, cco, m);
+        if (k == H.top) return -1;
+        ++i;
     }
 }
 
-// Root test of every group (collisions.hpp:33-34 via singleTraversal, traversal.hpp:35-45; the containedIn skip of
-// findHalos :88-89): one wave per group writes the group's frame for the root's children, with an empty ballot when
-// the group walks nothing. Block 0 also resets the level counters.
+// Level-0 bounds of each local leaf (empty when the leaf is inactive) and the root's witness: the lowest active leaf
+// whose search box overlaps the root box.
 template<class K, class T>
-__global__ __launch_bounds__(kHaloThreads) void haloRootKernel(HaloTree<K, T> t, const K* __restrict__ leaves,
-                                                               HaloFrame* __restrict__ out,
-                                                               unsigned long long* __restrict__ counters,
-                                                               int32_t numGroups)
+__global__ __launch_bounds__(256) void haloLeafBoundsKernel(HaloArgs<K, T> A, T* __restrict__ level0,
+                                                           HaloCounters* __restrict__ ctr)
 {
-    if (blockIdx.x == 0 && threadIdx.x < kHaloCounters)
-        counters[threadIdx.x] = threadIdx.x == 1 ? (unsigned long long)numGroups : 0ull;
-    const int lane    = laneId();
-    const int32_t g   = int32_t((size_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave);
-    if (g >= numGroups) return;
-    const int32_t leafIdx = t.firstNode + g * kWave + lane;
-    HaloLane<K, T> L;
-    L.init(t, leaves[t.firstNode], leaves[t.lastNode]);
-    bool active = leafIdx < t.lastNode;
+    const int32_t i = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= A.lastNode - A.firstNode) return;
+    T tC[3], tS[3];
+    load3(A.searchCenters, int64_t(A.firstNode) + i, tC);
+    load3(A.searchSizes, int64_t(A.firstNode) + i, tS);
+    const bool active = !boxContainedInRange<K, T>(A.leaves[A.firstNode], A.leaves[A.lastNode], tC, tS, A.box);
+    T* b = level0 + size_t(i) * 6;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        T slack  = (fabs(A.box.lim[2 * d]) + fabs(A.box.lim[2 * d + 1]) + A.box.len[d]) * boundsSlack<T>();
+        b[d]     = active ? (tC[d] - tS[d]) - slack : std::numeric_limits<T>::max();
+        b[3 + d] = active ? (tC[d] + tS[d]) + slack : std::numeric_limits<T>::lowest();
+    }
     if (active)
     {
-        for (int d = 0; d < 3; ++d)
-        {
-            L.tC[d] = t.searchCenters[size_t(leafIdx) * 3 + d];
-            L.tS[d] = t.searchSizes[size_t(leafIdx) * 3 + d];
-        }
-        active = !boxContainedInRange<K, T>(L.lowestKey, L.highestKey, L.tC, L.tS, t.box);
+        T c[3], s[3];
+        load3(A.nodeCenters, 0, c);
+        load3(A.nodeSizes, 0, s);
+        if (overlapBox(c, s, tC, tS, A.box)) atomicMin(&ctr->rootWitness, i);
     }
-    uint64_t rootMask = ballot(active && !L.excluded(t.prefixes[0]) && L.overlapBox(t.nodeCenters, t.nodeSizes, t.box));
-    int32_t rootChild = t.childOffsets[0];
-    if (rootMask != 0 && lane == 0) t.flags[0] = 1;
-    if (!descendable(0, 0, rootChild, t.numNodes)) rootMask = 0;
-    if (lane == 0) out[g] = HaloFrame{rootMask, g, rootChild};
 }
 
-// One level of the union walks: expands the frames in[0, min(*inCount, cap)) and appends the children's frames to
-// out, walking depth first the frames that find out full.
-template<class K, class T>
-__global__ __launch_bounds__(kHaloThreads) void haloLevelKernel(HaloTree<K, T> t, const K* __restrict__ leaves,
-                                                                const HaloFrame* __restrict__ in,
-                                                                const unsigned long long* __restrict__ inCount,
-                                                                HaloFrame* __restrict__ out,
-                                                                unsigned long long* __restrict__ outCount,
-                                                                unsigned long long cap)
+template<class T>
+__global__ __launch_bounds__(256) void haloBoundsReduceKernel(const T* __restrict__ in, int32_t nIn,
+                                                             T* __restrict__ out, int32_t nOut)
 {
-    __shared__ HaloShared sh;
-    const int lane = laneId();
-    const int w    = threadIdx.x / kWave;
-    const uint64_t n     = uniform64(*inCount < cap ? *inCount : cap);
-    const uint64_t waves = uint64_t(gridDim.x) * kHaloWaves;
-    const uint64_t gw    = uint64_t(xcdContiguousBlock(blockIdx.x, gridDim.x)) * kHaloWaves + uint64_t(w);
-    if (gw >= n) return;
-    HaloLane<K, T> L;
-    L.init(t, leaves[t.firstNode], leaves[t.lastNode]);
-    int staged = 0;
-
-    auto flush = [&]() {
-        waveSync();
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(outCount, (unsigned long long)staged);
-        base = uniform64(base);
-        if (lane < staged && base + lane < cap) out[base + lane] = sh.stage[w][lane];
-        for (uint64_t j = base < cap ? cap - base : 0; j < uint64_t(staged); ++j)
-            walkFrame(t, L, sh, w, lane, sh.stage[w][j]);
-        staged = 0;
-        waveSync();
-    };
-
-    for (uint64_t it = gw; it < n; it += waves)
+    const int32_t i = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= nOut) return;
+    T lo[3] = {std::numeric_limits<T>::max(), std::numeric_limits<T>::max(), std::numeric_limits<T>::max()};
+    T hi[3] = {std::numeric_limits<T>::lowest(), std::numeric_limits<T>::lowest(), std::numeric_limits<T>::lowest()};
+    for (int32_t j = 8 * i; j < 8 * i + 8 && j < nIn; ++j)
     {
-        const HaloFrame f = in[it];
-        const uint64_t pm = uniform64(f.mask);
-        if (pm == 0) continue;
-        const int32_t group = uniform(f.group);
-        const int32_t co    = uniform(f.co);
-        L.load(t, group, pm, lane);
-        uint64_t mk[8];
-        int32_t cco[8];
-        testChildren(t, L, co, pm, lane, mk, cco);
-        uint64_t mine = mk[0];
-        int32_t myCo  = cco[0];
 #pragma unroll
-        for (int k = 1; k < 8; ++k)
+        for (int d = 0; d < 3; ++d)
         {
-            mine = lane == k ? mk[k] : mine;
-            myCo = lane == k ? cco[k] : myCo;
+            T a = in[size_t(j) * 6 + d], b = in[size_t(j) * 6 + 3 + d];
+            lo[d] = a < lo[d] ? a : lo[d];
+            hi[d] = b > hi[d] ? b : hi[d];
         }
-        const bool emit    = lane < 8 && mine != 0 && descendable(co, lane, myCo, t.numNodes);
-        const uint64_t eb  = ballot(emit);
-        const int rank     = __builtin_popcountll(eb & ((uint64_t(1) << lane) - 1));
-        if (emit) sh.stage[w][staged + rank] = HaloFrame{mine, group, myCo};
-        staged += __builtin_popcountll(eb);
-        if (staged > kHaloStage - 8) flush();
     }
-    if (staged > 0) flush();
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        out[size_t(i) * 6 + d]     = lo[d];
+        out[size_t(i) * 6 + 3 + d] = hi[d];
+    }
+}
+
+__global__ void haloCountersInitKernel(HaloCounters* ctr, HaloBvhShape shape)
+{
+    int t = threadIdx.x;
+    if (t < 32)
+    {
+        ctr->start[t] = 0;
+        ctr->count[t] = 0;
+    }
+    if (t == 0)
+    {
+        ctr->rootWitness = INT32_MAX;
+#pragma unroll
+        for (int k = 0; k < kBvhMaxLevels; ++k)
+        {
+            ctr->bvh.offset[k] = shape.offset[k];
+            ctr->bvh.count[k]  = shape.count[k];
+        }
+        ctr->bvh.top = shape.top;
+    }
+}
+
+// The root (collisions.hpp:33-34 via singleTraversal, traversal.hpp:35-45): flagged iff it lies outside the local
+// key range and some active leaf overlaps it; if internal, it starts the frontier with its witness.
+template<class K, class T>
+__global__ void haloRootKernel(HaloArgs<K, T> A, int2* __restrict__ front, HaloCounters* __restrict__ ctr)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t w = ctr->rootWitness;
+    if (w == INT32_MAX || excludedPrefix(A.prefixes[0], A.leaves[A.firstNode], A.leaves[A.lastNode])) return;
+    A.flags[0]       = 1;
+    const int32_t co = A.childOffsets[0];
+    if (co > 0 && co + 8 <= A.numNodes)
+    {
+        front[0]     = make_int2(0, w);
+        ctr->count[0] = 1;
+    }
+}
+
+// Depth d of the walk: tests the 8 children of each frontier node of depth d, flags them, and appends the flagged
+// internal children with their witnesses as the frontier of depth d + 1.
+template<class K, class T>
+__global__ __launch_bounds__(kHaloThreads) void haloLevelKernel(HaloArgs<K, T> A, const T* __restrict__ bvh,
+                                                                int2* __restrict__ front, uint32_t capacity,
+                                                                HaloCounters* __restrict__ ctr, int depth)
+{
+    const uint32_t begin = ctr->start[depth];
+    const uint32_t n     = begin >= capacity ? 0 : std::min(ctr->count[depth], capacity - begin);
+    const uint32_t next  = begin + n; // the next level's entries follow this level's
+    const K lowestKey    = A.leaves[A.firstNode];
+    const K highestKey   = A.leaves[A.lastNode];
+    const HaloBvhShape& H = ctr->bvh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->start[depth + 1] = next;
+    const uint64_t total  = uint64_t(n) * 8;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const int lane        = laneId();
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x - lane); base < total; base += stride)
+    {
+        const uint64_t t = base + lane;
+        bool emit        = false;
+        int32_t c = 0, wit = -1;
+        if (t < total)
+        {
+            const int2 e = front[begin + t / 8];
+            c            = A.childOffsets[e.x] + int32_t(t % 8);
+            if (!excludedPrefix(A.prefixes[c], lowestKey, highestKey))
+            {
+                T cb[3], sb[3], tC[3], tS[3];
+                load3(A.nodeCenters, c, cb);
+                load3(A.nodeSizes, c, sb);
+                load3(A.searchCenters, int64_t(A.firstNode) + e.y, tC);
+                load3(A.searchSizes, int64_t(A.firstNode) + e.y, tS);
+                wit = overlapBox(cb, sb, tC, tS, A.box) ? e.y : findWitness(A, bvh, H, e.x, cb, sb);
+                if (wit >= 0)
+                {
+                    A.flags[c]        = 1;
+                    const int32_t cco = A.childOffsets[c];
+                    emit              = cco > c && cco + 8 <= A.numNodes;
+                }
+            }
+        }
+        const uint64_t eb = ballot(emit);
+        if (eb == 0) continue;
+        uint32_t pos = 0;
+        if (lane == 0) pos = atomicAdd(&ctr->count[depth + 1], uint32_t(__builtin_popcountll(eb)));
+        pos            = uniform(pos);
+        const uint32_t slot = next + pos + uint32_t(__builtin_popcountll(eb & ((uint64_t(1) << lane) - 1)));
+        if (emit && slot < capacity) front[slot] = make_int2(c, wit);
+    }
 }
 
 template<class T>
@@ -413,42 +433,32 @@ static hipcstone_status boundingBox(hipcstone_stream s, const T* x, const T* y, 
     return HIPCSTONE_OK;
 }
 
-// Workspace of findHalos: the level counters, then two frame queues of equal capacity.
-inline unsigned long long haloQueueCapacity(size_t workspaceBytes)
+// Workspace of findHalos: counters, the frontier (at most one entry per node: an internal node enters it at most
+// once, from its parent), then the bounds hierarchy.
+struct HaloLayout
 {
-    size_t head = kHaloCounters * sizeof(unsigned long long);
-    return workspaceBytes <= head ? 0 : (workspaceBytes - head) / (2 * sizeof(HaloFrame));
-}
+    size_t frontOffset, bvhOffset, bytes;
+    int32_t count[kBvhMaxLevels], offset[kBvhMaxLevels], top;
+};
 
-inline size_t haloWorkspaceBytes(size_t queueCapacity)
+inline HaloLayout haloLayout(int32_t numNodes, int32_t numLocal, size_t realBytes)
 {
-    return kHaloCounters * sizeof(unsigned long long) + 2 * queueCapacity * sizeof(HaloFrame);
-}
-
-// Recommended size: queues of two frames per tree node (frames beyond the capacity are walked depth first).
-inline size_t findHalosWorkspaceSize(int32_t numNodes)
-{
-    size_t cap = std::min(std::max(size_t(2) * size_t(std::max(numNodes, 0)), size_t(1) << 14), size_t(1) << 25);
-    return haloWorkspaceBytes(cap);
-}
-
-// Resident blocks of the level kernel over the whole GPU, a multiple of the XCD count.
-template<class K, class T>
-static unsigned haloLevelGrid()
-{
-    static unsigned cached[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cached[dev]) return cached[dev];
-    int perCu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, haloLevelKernel<K, T>, kHaloThreads, 0) != hipSuccess ||
-        perCu <= 0)
-        perCu = 2;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-    unsigned g  = unsigned(perCu) * unsigned(cus);
-    g           = std::min(std::max(g, kXcds), kHaloBlocks * 4);
-    cached[dev] = (g + kXcds - 1) / kXcds * kXcds;
-    return cached[dev];
+    HaloLayout L{};
+    auto align   = [](size_t v) { return (v + 255) / 256 * 256; };
+    L.frontOffset = kHaloHeadBytes;
+    L.bvhOffset   = align(L.frontOffset + size_t(std::max(numNodes, 1)) * sizeof(int2));
+    int32_t n = std::max(numLocal, 1), total = 0, k = 0;
+    for (;; ++k)
+    {
+        L.count[k]  = n;
+        L.offset[k] = total;
+        total += n;
+        if (n == 1 || k + 1 == kBvhMaxLevels) break;
+        n = (n + 7) / 8;
+    }
+    L.top   = k;
+    L.bytes = align(L.bvhOffset + size_t(total) * 6 * realBytes);
+    return L;
 }
 
 template<class K, class T>
@@ -464,33 +474,48 @@ static hipcstone_status findHalos(hipcstone_stream s, const K* prefixes, const i
         return invalid("findHalos: null pointer");
     if (numNodes <= 0) return invalid("findHalos: numNodes must be positive");
     if (firstNode < 0) return invalid("findHalos: negative first node");
-    const size_t groups = (size_t(lastNode - firstNode) + kWave - 1) / kWave;
-    void* owned         = nullptr;
+    const HaloLayout lay = haloLayout(numNodes, lastNode - firstNode, sizeof(T));
+    void* owned          = nullptr;
     if (!workspace)
     {
-        workspaceBytes = std::max(findHalosWorkspaceSize(numNodes), haloWorkspaceBytes(std::max(groups, size_t(64))));
-        HC_CHECK(hipMallocAsync(&owned, workspaceBytes, stream(s)));
+        HC_CHECK(hipMallocAsync(&owned, lay.bytes, stream(s)));
         workspace = owned;
     }
-    const unsigned long long cap = haloQueueCapacity(workspaceBytes);
-    if (cap < std::max(groups, size_t(kHaloStage)))
-        return invalid("findHalos: workspace smaller than 256 + 32 * max(64, ceil((last - first) / 64)) bytes");
+    else if (workspaceBytes < lay.bytes)
+        return invalid("findHalos: workspace smaller than hipcstone_find_halos_workspace_size()");
 
-    auto* counters = static_cast<unsigned long long*>(workspace);
-    auto* queueA   = reinterpret_cast<HaloFrame*>(counters + kHaloCounters);
-    auto* queueB   = queueA + cap;
-    HaloTree<K, T> t{prefixes, childOffsets, nodeCenters, nodeSizes, searchCenters, searchSizes, *box,
-                     firstNode,  lastNode,    numNodes,   flags};
-    haloRootKernel<K, T><<<ceilDiv(groups * kWave, kHaloThreads), kHaloThreads, 0, stream(s)>>>(
-        t, leaves, queueA, counters, int32_t(groups));
-    HC_CHECK_LAUNCH("haloRootKernel");
-    const unsigned grid = haloLevelGrid<K, T>();
-    for (int d = 1; d <= int(maxLevel<K>()); ++d)
+    auto* base  = static_cast<char*>(workspace);
+    auto* ctr   = reinterpret_cast<HaloCounters*>(base);
+    auto* front = reinterpret_cast<int2*>(base + lay.frontOffset);
+    T* bvh      = reinterpret_cast<T*>(base + lay.bvhOffset);
+    HaloBvhShape shape{};
+    for (int k = 0; k < kBvhMaxLevels; ++k)
     {
-        HaloFrame* in  = (d & 1) ? queueA : queueB;
-        HaloFrame* out = (d & 1) ? queueB : queueA;
-        haloLevelKernel<K, T><<<grid, kHaloThreads, 0, stream(s)>>>(t, leaves, in, counters + d, out,
-                                                                    counters + d + 1, cap);
+        shape.offset[k] = lay.offset[k];
+        shape.count[k]  = k <= lay.top ? lay.count[k] : 0;
+    }
+    shape.top = lay.top;
+    HaloArgs<K, T> A{prefixes, childOffsets, parents, nodeCenters, nodeSizes, searchCenters, searchSizes,
+                     leaves,   *box,         firstNode, lastNode,  numNodes,  flags};
+
+    haloCountersInitKernel<<<1, 64, 0, stream(s)>>>(ctr, shape);
+    HC_CHECK_LAUNCH("haloCountersInitKernel");
+    const int32_t numLocal = lastNode - firstNode;
+    haloLeafBoundsKernel<K, T><<<ceilDiv(size_t(numLocal), 256), 256, 0, stream(s)>>>(A, bvh, ctr);
+    HC_CHECK_LAUNCH("haloLeafBoundsKernel");
+    for (int k = 0; k < lay.top; ++k)
+    {
+        haloBoundsReduceKernel<T><<<ceilDiv(size_t(lay.count[k + 1]), 256), 256, 0, stream(s)>>>(
+            bvh + size_t(lay.offset[k]) * 6, lay.count[k], bvh + size_t(lay.offset[k + 1]) * 6,
+            lay.count[k + 1]);
+        HC_CHECK_LAUNCH("haloBoundsReduceKernel");
+    }
+    haloRootKernel<K, T><<<1, 64, 0, stream(s)>>>(A, front, ctr);
+    HC_CHECK_LAUNCH("haloRootKernel");
+    for (int d = 0; d < int(maxLevel<K>()); ++d)
+    {
+        haloLevelKernel<K, T><<<kHaloBlocks, kHaloThreads, 0, stream(s)>>>(A, bvh, front, uint32_t(numNodes), ctr,
+                                                                          d);
         HC_CHECK_LAUNCH("haloLevelKernel");
     }
     if (owned) HC_CHECK(hipFreeAsync(owned, stream(s)));
@@ -515,7 +540,10 @@ extern "C"
 HIPCSTONE_DEF_BBOX(f32, float)
 HIPCSTONE_DEF_BBOX(f64, double)
 
-size_t hipcstone_find_halos_workspace_size(int32_t num_nodes) { return findHalosWorkspaceSize(num_nodes); }
+size_t hipcstone_find_halos_workspace_size(int32_t num_nodes, int32_t num_local_leaves, int32_t real_bytes)
+{
+    return haloLayout(num_nodes, num_local_leaves, real_bytes == 4 ? 4 : 8).bytes;
+}
 
 #define HIPCSTONE_DEF_HALOS(KS, K, RS, T)                                                                              \
     hipcstone_status hipcstone_find_halos_##KS##_##RS(                                                                 \

@@ -194,7 +194,6 @@ public:
         if (auto e = assignCnt_.reserve(R * sizeof(uint32_t))) return e;
         if (auto e = assignWs_.reserve(hipcstone_sfc_assignment_workspace_size(L))) return e;
         if (auto e = flags_.reserve(size_t(N))) return e;
-        if (auto e = haloWs_.reserve(hipcstone_find_halos_workspace_size(N))) return e;
         if (auto e = searchC_.reserve(size_t(L) * 3 * sizeof(T))) return e;
         if (auto e = searchS_.reserve(size_t(L) * 3 * sizeof(T))) return e;
         if (auto e = localLayout_.reserve((size_t(L) + 1) * sizeof(uint32_t))) return e;
@@ -219,6 +218,8 @@ public:
         HC_CHECK(hipStreamSynchronize(stream(s)));
         first_ = range[0];
         last_  = range[1];
+        if (auto e = haloWs_.reserve(hipcstone_find_halos_workspace_size(N, last_ - first_, int32_t(sizeof(T)))))
+            return e;
         HC_STAGE(8, A::discover(s, prefixes_.as<K>(), childOffsets_.as<int32_t>(), parents_.as<int32_t>(),
                                 leafToInternalLeaves, centers_.as<T>(), sizes_.as<T>(), tree_[treeCur_].template as<K>(),
                                 N, x, y, z, h, layout_.as<uint32_t>(), first_, last_, &box_, ext, searchC_.as<T>(),

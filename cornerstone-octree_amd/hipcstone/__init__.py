@@ -381,15 +381,16 @@ def compute_bounding_box(x, y, z, h, layout, first, last, scale, centers, sizes,
              ctype(scale), _p(centers), _p(sizes)), "compute_bounding_box")
 
 
-def find_halos_workspace_size(num_nodes):
+def find_halos_workspace_size(num_nodes, num_local_leaves, real_dtype):
     _lib.hipcstone_find_halos_workspace_size.restype = C.c_size_t
-    return int(_lib.hipcstone_find_halos_workspace_size(C.c_int32(num_nodes)))
+    return int(_lib.hipcstone_find_halos_workspace_size(C.c_int32(num_nodes), C.c_int32(num_local_leaves),
+                                                        C.c_int32(np.dtype(real_dtype).itemsize)))
 
 
 def find_halos(prefixes, child_offsets, parents, centers, sizes, leaves, search_centers, search_sizes, box, first,
                last, num_nodes, flags, workspace=None, workspace_bytes=None, stream=None):
-    """findHalos (collisions.hpp:68-94). workspace: a DeviceArray (its size, or workspace_bytes of it, bounds the
-    walk's frame queues) or None for a stream-ordered allocation inside the call."""
+    """findHalos (collisions.hpp:68-94). workspace: a DeviceArray of at least find_halos_workspace_size() bytes, or
+    None for a stream-ordered allocation inside the call."""
     f = _fn(f"hipcstone_find_halos_{_KS[prefixes.dtype]}_{_RS[centers.dtype]}")
     wsb = 0 if workspace is None else (workspace.nbytes if workspace_bytes is None else int(workspace_bytes))
     _check(f(_stream(stream), _p(prefixes), _p(child_offsets), _p(parents), _p(centers), _p(sizes), _p(leaves),

@@ -252,11 +252,11 @@ HIPCSTONE_DECL_BBOX(f64, double)
 /* ---- a14: halo discovery. Replaces findHalosGpu(exec, prefixes, childOffsets, parents, nodeCenters, nodeSizes,
  * leaves, searchCenters, searchSizes, box, firstNode, lastNode, collisionFlags), traversal/collisions_gpu.h:46-59
  * (CPU collisions.hpp:68-94). flags (node-indexed) are set to 1 for halo nodes and never cleared. num_nodes is the
- * size of the linked tree (bounds every walk). workspace holds the walk's frame queues: any size of at least
- * 256 + 32 * max(64, ceil((last_node - first_node) / 64)) bytes gives the same flags (frames that do not fit are
- * walked depth first by one wave); hipcstone_find_halos_workspace_size(num_nodes) is large enough to balance the
- * whole walk over the GPU. A NULL workspace is allocated and freed on the stream (hipMallocAsync). ---- */
-size_t hipcstone_find_halos_workspace_size(int32_t num_nodes);
+ * size of the linked tree (bounds every walk). workspace (at least
+ * hipcstone_find_halos_workspace_size(num_nodes, last_node - first_node, sizeof(T)) bytes) holds the walk's counters,
+ * frontier and the bounds of the search boxes; a NULL workspace is allocated and freed on the stream
+ * (hipMallocAsync). ---- */
+size_t hipcstone_find_halos_workspace_size(int32_t num_nodes, int32_t num_local_leaves, int32_t real_bytes);
 #define HIPCSTONE_DECL_HALOS(KS, K, RS, T)                                                                             \
     hipcstone_status hipcstone_find_halos_##KS##_##RS(                                                                 \
         hipcstone_stream s, const K* prefixes, const int32_t* child_offsets, const int32_t* parents,                  \

@@ -298,17 +298,17 @@ def test_bbox_and_halos(hip, oracle, kd, T):
                             rs, obox, first, last)
     assert ref.sum() > 0
     num_nodes = p["prefixes"].size
-    groups = (last - first + 63) // 64
+    wsb = hip.find_halos_workspace_size(num_nodes, last - first, T)
     dc, dsz = dev(hip, p["centers"].reshape(-1)), dev(hip, p["sizes"].reshape(-1))
-    # None: allocated in the call; the smallest workspace sends nearly every frame down the depth-first path; the
-    # recommended one balances everything through the level queues
-    for ws in (None, hip.DeviceArray(256 + 32 * max(64, groups), np.uint8),
-               hip.DeviceArray(hip.find_halos_workspace_size(num_nodes), np.uint8)):
+    # None: allocated in the call; otherwise a caller workspace, reused (the walk resets its own counters)
+    ws = hip.DeviceArray(wsb, np.uint8)
+    ws.fill(0xFF)
+    for w in (None, ws, ws):
         flags = hip.DeviceArray(num_nodes, np.uint8)
         flags.fill(0)
         hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dc, dsz, d["leaves"], dsc, dss, gbox, first,
-                       last, num_nodes, flags, workspace=ws)
-        assert np.array_equal(flags.download(), ref), None if ws is None else ws.nbytes
+                       last, num_nodes, flags, workspace=w)
+        assert np.array_equal(flags.download(), ref)
     with pytest.raises(hip.HipcstoneError):
         hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dc, dsz, d["leaves"], dsc, dss, gbox, first,
-                       last, num_nodes, flags, workspace=hip.DeviceArray(256 + 32 * max(64, groups) - 32, np.uint8))
+                       last, num_nodes, flags, workspace=ws, workspace_bytes=wsb - 1)
