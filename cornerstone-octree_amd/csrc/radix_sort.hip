@@ -12,7 +12,14 @@
 //   3. publishes its per-digit counts and resolves its global digit offsets by decoupled look-back
 //      (one thread per digit, status word = 2-bit flag | 30-bit count),
 //   4. reorders the tile in LDS and writes each digit run contiguously.
-// The last pass optionally writes the permutation and gathers x,y,z,h through it.
+// The last pass optionally writes the permutation and gathers x,y,z,h through it. Gathering 4 separate arrays at
+// random positions fetches 4 memory lines per particle; when the input is far from key order (more than n/8
+// descents, counted by the histogram kernel), x,y,z,h are first packed into one 4-value record per particle so that
+// the gather fetches one line per particle. Input already close to key order (a previous step's output) gathers the
+// separate arrays directly, which is coalesced. Both read the same values, so the result does not depend on the
+// choice.
+#include <type_traits>
+
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -39,13 +46,15 @@ using namespace sortcfg;
 
 // Workspace layout (bytes, 256-aligned pieces):
 //   hist    [kMaxPasses][kRadix] u32   digit histograms, then exclusive digit offsets
-//   counter [kMaxPasses] u32 (padded) tile-id counters
+//   counter [kMaxPasses + 1] u32 (padded) tile-id counters, then the input's descent count
 //   status  [kMaxPasses][numTiles][kRadix] u32 look-back words
+//   packed  [n][4] x 8 bytes           x,y,z,h records (sort-gather workspaces only)
 struct SortWorkspace
 {
     uint32_t* hist;
     uint32_t* counter;
     uint32_t* status;
+    void* packed;
     size_t bytes;
 };
 
@@ -53,7 +62,7 @@ static size_t alignUp(size_t v) { return (v + 255) & ~size_t(255); }
 
 static size_t numTiles(size_t n) { return (n + kTile - 1) / kTile; }
 
-static SortWorkspace carveWorkspace(void* base, size_t n, int passes)
+static SortWorkspace carveWorkspace(void* base, size_t n, int passes, bool packed)
 {
     SortWorkspace w{};
     size_t off = 0;
@@ -61,16 +70,21 @@ static SortWorkspace carveWorkspace(void* base, size_t n, int passes)
     w.hist     = reinterpret_cast<uint32_t*>(b + off);
     off += alignUp(sizeof(uint32_t) * kMaxPasses * kRadix);
     w.counter = reinterpret_cast<uint32_t*>(b + off);
-    off += alignUp(sizeof(uint32_t) * kMaxPasses);
+    off += alignUp(sizeof(uint32_t) * (kMaxPasses + 1));
     w.status = reinterpret_cast<uint32_t*>(b + off);
     off += alignUp(sizeof(uint32_t) * size_t(passes) * numTiles(n) * kRadix);
+    if (packed)
+    {
+        w.packed = b + off;
+        off += alignUp(n * 4 * sizeof(double));
+    }
     w.bytes = off;
     return w;
 }
 
-static size_t workspaceBytes(size_t n, int passes)
+static size_t workspaceBytes(size_t n, int passes, bool packed)
 {
-    return carveWorkspace(nullptr, n, passes).bytes;
+    return carveWorkspace(nullptr, n, passes, packed).bytes;
 }
 
 __device__ inline uint32_t atomicLoadRelaxed(const uint32_t* p)
@@ -84,13 +98,16 @@ __device__ inline void atomicStoreRelaxed(uint32_t* p, uint32_t v)
 }
 
 // Digit histograms of all passes [0, numPasses) in one read of the keys. hist must be zeroed.
+// With descents set, also counts the positions i with keys[i] > keys[i + 1].
 template<class K>
 __global__ __launch_bounds__(kThreads) void histogramKernel(const K* __restrict__ keys, size_t n, int numPasses,
-                                                            uint32_t* __restrict__ hist)
+                                                            uint32_t* __restrict__ hist, uint32_t* __restrict__ descents)
 {
     __shared__ uint32_t sh[kMaxPasses * kRadix];
+    __shared__ uint32_t shDescents;
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kThreads)
         sh[i] = 0;
+    if (threadIdx.x == 0) shDescents = 0;
     __syncthreads();
 
     const int lane = laneId();
@@ -102,6 +119,11 @@ __global__ __launch_bounds__(kThreads) void histogramKernel(const K* __restrict_
         K key        = valid ? keys[i] : K(0);
         uint64_t act = ballot(valid);
         if (act == 0) continue;
+        if (descents)
+        {
+            uint64_t down = ballot(i + 1 < n && key > keys[i + 1]);
+            if (down && lane == __builtin_ctzll(act)) atomicAdd(&shDescents, uint32_t(__popcll(down)));
+        }
         int first = __builtin_ctzll(act);
         for (int p = 0; p < numPasses; ++p)
         {
@@ -121,6 +143,27 @@ __global__ __launch_bounds__(kThreads) void histogramKernel(const K* __restrict_
     {
         if (sh[i]) atomicAdd(&hist[i], sh[i]);
     }
+    if (descents && threadIdx.x == 0 && shDescents) atomicAdd(descents, shDescents);
+}
+
+// x,y,z,h records of the particles in input order, written only when the input is far from key order (the
+// descent count is final: this kernel runs after the histogram kernel on the same stream).
+template<class T>
+__global__ __launch_bounds__(256) void packRecordsKernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                         const T* __restrict__ z, const T* __restrict__ h, size_t n,
+                                                         const uint32_t* __restrict__ descents, uint32_t threshold,
+                                                         T* __restrict__ packed)
+{
+    if (*descents <= threshold) return;
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using V4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+    V4 r;
+    r.x                               = x[i];
+    r.y                               = y[i];
+    r.z                               = z[i];
+    r.w                               = h ? h[i] : T(0);
+    reinterpret_cast<V4*>(packed)[i] = r;
 }
 
 // Exclusive scan of each pass's 256-bin histogram, in place. One block of kRadix threads.
@@ -156,6 +199,9 @@ struct GatherArgs
     T* yOut;
     T* zOut;
     T* hOut;
+    const T* packed;           // x,y,z,h records, used when *descents > threshold
+    const uint32_t* descents;
+    uint32_t threshold;
 };
 
 // ValueMode: 0 = keys only, 1 = values read from valuesIn, 2 = values are the input positions (iota)
@@ -310,6 +356,8 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
     __syncthreads();
 
     // ---- write out: consecutive LDS slots of one digit go to consecutive global slots ----
+    bool usePacked = false;
+    if constexpr (GatherOn) usePacked = g.packed && *g.descents > g.threshold;
     for (uint32_t i = tid; i < validInTile; i += kThreads)
     {
         K key        = shKeys[i];
@@ -322,10 +370,22 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
             valuesOut[dst] = v;
             if constexpr (GatherOn)
             {
-                g.xOut[dst] = g.xIn[v];
-                g.yOut[dst] = g.yIn[v];
-                g.zOut[dst] = g.zIn[v];
-                if (g.hIn) g.hOut[dst] = g.hIn[v];
+                if (usePacked)
+                {
+                    using V4 = typename std::conditional<sizeof(GT) == 4, float4, double4>::type;
+                    V4 r     = reinterpret_cast<const V4*>(g.packed)[v];
+                    g.xOut[dst] = r.x;
+                    g.yOut[dst] = r.y;
+                    g.zOut[dst] = r.z;
+                    if (g.hIn) g.hOut[dst] = r.w;
+                }
+                else
+                {
+                    g.xOut[dst] = g.xIn[v];
+                    g.yOut[dst] = g.yIn[v];
+                    g.zOut[dst] = g.zIn[v];
+                    if (g.hIn) g.hOut[dst] = g.hIn[v];
+                }
             }
         }
     }
@@ -368,20 +428,33 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
     if (n > kValueMask) return invalid("radix sort: more than 2^30-1 keys");
     if (endBit <= 0 || endBit > int(8 * sizeof(K))) endBit = int(8 * sizeof(K));
     int passes = numPassesFor<K>(endBit);
-    size_t need = workspaceBytes(n, passes);
+    const bool pack = g != nullptr;
+    size_t need     = workspaceBytes(n, passes, pack);
     if (!workspace || wsBytes < need) return invalid("radix sort: workspace too small");
     if (!keys || !keyBuf) return invalid("radix sort: null key buffer");
     if (valueMode != 0 && (!values || !valueBuf)) return invalid("radix sort: null value buffer");
 
-    SortWorkspace w = carveWorkspace(workspace, n, passes);
+    SortWorkspace w = carveWorkspace(workspace, n, passes, pack);
     size_t tiles    = numTiles(n);
     if (auto st = zeroU32(w.hist, size_t(kMaxPasses) * kRadix, s)) return st;
-    if (auto st = zeroU32(w.counter, kMaxPasses, s)) return st;
+    if (auto st = zeroU32(w.counter, kMaxPasses + 1, s)) return st;
     if (auto st = zeroU32(w.status, size_t(passes) * tiles * kRadix, s)) return st;
 
     unsigned hb = unsigned(std::min<size_t>(kHistBlocks, (n + kThreads - 1) / kThreads));
-    histogramKernel<K><<<hb, kThreads, 0, s>>>(keys, n, passes, w.hist);
+    uint32_t* descents = pack ? w.counter + kMaxPasses : nullptr;
+    histogramKernel<K><<<hb, kThreads, 0, s>>>(keys, n, passes, w.hist, descents);
     HC_CHECK_LAUNCH("histogramKernel");
+    GatherArgs<GT> gp{};
+    if (pack)
+    {
+        gp           = *g;
+        gp.packed    = static_cast<const GT*>(w.packed);
+        gp.descents  = descents;
+        gp.threshold = uint32_t(n / 8);
+        packRecordsKernel<GT><<<ceilDiv(n, 256), 256, 0, s>>>(g->xIn, g->yIn, g->zIn, g->hIn, n, descents,
+                                                             gp.threshold, static_cast<GT*>(w.packed));
+        HC_CHECK_LAUNCH("packRecordsKernel");
+    }
     scanHistogramKernel<<<1, kRadix, 0, s>>>(w.hist, passes);
     HC_CHECK_LAUNCH("scanHistogramKernel");
 
@@ -423,13 +496,13 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
                 src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather);
         else if (mode == 2 && gatherNow)
             onesweepPassKernel<K, 2, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, *g);
+                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, gp);
         else if (!gatherNow)
             onesweepPassKernel<K, 1, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
                 src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather);
         else
             onesweepPassKernel<K, 1, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, *g);
+                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, gp);
         HC_CHECK_LAUNCH("onesweepPassKernel");
         src  = dst;
         vsrc = vdst;
@@ -438,9 +511,9 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
 }
 
 template<class K>
-static size_t sortWorkspaceSize(size_t n)
+static size_t sortWorkspaceSize(size_t n, bool packed)
 {
-    return workspaceBytes(n, numPassesFor<K>(int(8 * sizeof(K))));
+    return workspaceBytes(n, numPassesFor<K>(int(8 * sizeof(K))), packed);
 }
 
 template<class T>
@@ -460,7 +533,7 @@ static hipcstone_status sortGather(hipcstone_stream s, K* keys, uint32_t* perm, 
 {
     if (n == 0) return HIPCSTONE_OK;
     if (!xIn || !yIn || !zIn || !xOut || !yOut || !zOut || (hIn && !hOut)) return invalid("sortGather: null array");
-    GatherArgs<T> g{xIn, yIn, zIn, hIn, xOut, yOut, zOut, hOut};
+    GatherArgs<T> g{xIn, yIn, zIn, hIn, xOut, yOut, zOut, hOut, nullptr, nullptr, 0};
     return radixSort<K, T>(stream(s), keys, perm, n, keyBuf, valueBuf, ws, wsBytes, int(8 * sizeof(K)), 2, &g);
 }
 
@@ -471,10 +544,10 @@ using namespace hipcstone;
 extern "C"
 {
 
-size_t hipcstone_sort_by_key_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n); }
-size_t hipcstone_sort_by_key_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n); }
-size_t hipcstone_sort_gather_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n); }
-size_t hipcstone_sort_gather_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n); }
+size_t hipcstone_sort_by_key_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n, false); }
+size_t hipcstone_sort_by_key_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n, false); }
+size_t hipcstone_sort_gather_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n, true); }
+size_t hipcstone_sort_gather_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n, true); }
 
 hipcstone_status hipcstone_sort_by_key_u32(hipcstone_stream s, uint32_t* keys, uint32_t* values, size_t n,
                                            uint32_t* key_buf, uint32_t* value_buf, void* workspace,

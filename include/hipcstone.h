@@ -134,7 +134,9 @@ hipcstone_status hipcstone_sort_by_key_u64(hipcstone_stream s, uint64_t* keys, u
 /* ---- a4+a5 fused: keys sorted with the permutation iota(0..n) and x,y,z,h gathered by it, the sequence of
  * sequence() + sortByKey() + gather() x4 (primitives/primitives_gpu.h:100-155, :48-56; domain/domain.hpp:187-188).
  * keys/perm are sorted in place (perm is write-only: the identity is generated on the fly); x,y,z,h are read from
- * the *_in arrays and written in sorted order to *_out. h may be NULL (then h_out is ignored). */
+ * the *_in arrays and written in sorted order to *_out. h may be NULL (then h_out is ignored). The workspace also
+ * holds 32 bytes per key: when the input keys are far from sorted (more than n/8 descents), x,y,z,h are packed into
+ * one record per particle there before the gather. */
 size_t hipcstone_sort_gather_workspace_size_u32(size_t n);
 size_t hipcstone_sort_gather_workspace_size_u64(size_t n);
 #define HIPCSTONE_DECL_SORTGATHER(KS, K, RS, T)                                                                        \

@@ -85,19 +85,34 @@ def test_sort_partial_bits_and_keys_only(hip, end_bit):
 
 @pytest.mark.parametrize("kd", KDS)
 @pytest.mark.parametrize("T", TS)
-def test_sort_gather(hip, oracle, kd, T):
+@pytest.mark.parametrize("order", ["random", "near_sorted"])
+def test_sort_gather(hip, oracle, kd, T, order):
+    """Random input order gathers through packed x,y,z,h records, input close to key order gathers the arrays
+    directly; both must give the oracle's order and values."""
     x, y, z, h = plummer_input(300001, T)
     gbox, obox = boxes(hip, T, -1.0, 1.0)
+    if order == "near_sorted":
+        _, p0 = oracle.sort_by_key(oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT))
+        p0 = p0.copy()
+        sw = np.random.default_rng(3).integers(0, x.size - 1, size=2000)
+        p0[sw], p0[sw + 1] = p0[sw + 1], p0[sw].copy()
+        x, y, z, h = (a[p0] for a in (x, y, z, h))
     keys = oracle.compute_sfc_keys(x, y, z, obox, kd, HILBERT)
     ks, perm = oracle.sort_by_key(keys)
-    dk = dev(hip, keys)
-    dp = hip.DeviceArray(x.size, np.uint32)
-    outs = [hip.DeviceArray(x.size, T) for _ in range(4)]
-    hip.sort_gather(dk, dp, dev(hip, x), dev(hip, y), dev(hip, z), dev(hip, h), *outs)
-    assert np.array_equal(dk.download(), ks)
-    assert np.array_equal(dp.download(), perm)
-    for o, a in zip(outs, (x, y, z, h)):
-        assert np.array_equal(o.download(), a[perm])
+    for hh in (h, None):
+        dk = dev(hip, keys)
+        dp = hip.DeviceArray(x.size, np.uint32)
+        outs = [hip.DeviceArray(x.size, T) for _ in range(4)]
+        outs[3].fill(0)
+        hip.sort_gather(dk, dp, dev(hip, x), dev(hip, y), dev(hip, z), None if hh is None else dev(hip, hh), *outs)
+        assert np.array_equal(dk.download(), ks)
+        assert np.array_equal(dp.download(), perm)
+        for o, a in zip(outs[:3], (x, y, z)):
+            assert np.array_equal(o.download(), a[perm])
+        if hh is None:
+            assert not outs[3].download().any()  # h_out is not written without h
+        else:
+            assert np.array_equal(outs[3].download(), h[perm])
 
 
 # ---------------------------------------------------------------- leaf array
