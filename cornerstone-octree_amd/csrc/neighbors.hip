@@ -10,6 +10,8 @@
 //  * Walk: depth-first with an explicit per-wave stack in LDS. Descending into a node evaluates its 8 children at
 //    once (their centers/sizes are contiguous, so a few wide scalar loads fetch them) and stores, per child, the
 //    ballot of lanes whose own predicate chain passed: a lane is tested at a child only if it passed the parent.
+//    The frame also keeps each child's first child (or, for a leaf, its leaf index), fetched in the same batch of
+//    loads, so each step of the walk waits on one memory round trip instead of two.
 //    The visited leaf set of every lane is therefore exactly the CPU's, not the superset a group-box walk gives.
 //    Children are taken in order 0..7, so leaves come out in the CPU's visiting order (ascending SFC order).
 //  * Leaves and their lane masks go to a per-wave list, drained in visiting order, so every target's neighbors come
@@ -19,6 +21,7 @@
 //    on typical leaves). A ballot ranks each hit among its target's hits in ascending j, so every target's hits of a
 //    leaf are stored as one contiguous run of its row.
 //  * Every loop is bounded: the walk by 2 * numNodes + 8 steps and the stack depth by the key's level count.
+#include <limits>
 #include <type_traits>
 
 #include "common.hpp"
@@ -66,10 +69,10 @@ struct alignas(4 * sizeof(T)) Particle4
 template<class T>
 struct alignas(4 * sizeof(T)) TargetRec
 {
-    Particle4<T> p;  // x, y, z, 4h^2
-    uint32_t count;  // neighbors found so far
-    uint32_t meta;   // periodic-image flags (bits 0-2) | lane of the target (bits 8-13)
-    uint32_t rowOff; // element offset of the output row from the wave's first row
+    Particle4<T> p; // x, y, z, 4h^2
+    uint32_t wpos;  // element offset (from the wave's first row) of the target's next neighbor slot
+    uint32_t limit; // element offset of the end of the target's row
+    uint32_t meta;  // periodic-image flags (bits 0-2) | lane of the target (bits 8-13)
     uint32_t pad;
 };
 
@@ -81,8 +84,10 @@ struct NbShared
     int32_t next[kNbWaves][kMaxDepth];
     int32_t listLeaf[kNbWaves][kLeafList];
     uint64_t listMask[kNbWaves][kLeafList];
-    TargetRec<T> ctgt[kNbWaves][kWave]; // the current leaf's active targets, compacted in lane order
-    uint32_t dummy[kNbWaves][kWave];    // sink of the count writes of lanes that own no count
+    // the current leaf's active targets, compacted in lane order: wave w's records are [w * 64, w * 64 + 64); the
+    // 64 records past the last wave's are padding, so slot indexes up to 127 read inside the array (such reads
+    // belong to slots without a target and are masked out) and need no clamp
+    TargetRec<T> ctgt[kNbWaves * kWave + kWave];
 };
 
 template<class T, bool AnyPbc>
@@ -141,12 +146,13 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         return d0 * d0 + (d1 * d1 + d2 * d2);
     };
 
-    // per-target state: the lane of target t keeps its running count in a register; while a leaf is drained the
-    // leaf's active targets are copied, compacted, into LDS for the lanes that test them
-    const TargetRec<T> myRec{Particle4<T>{xi, yi, zi, radiusSq}, 0u,
-                             uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)) | (uint32_t(lane) << 8),
-                             uint32_t(lane) * ngmax, 0u};
-    uint32_t myCount = 0;
+    // per-target state: the lane of target t keeps its next row slot in a register (the count is wpos - rowOff, all
+    // neighbors counted, only the first ngmax stored); while a leaf is drained the leaf's active targets are copied,
+    // compacted, into LDS for the lanes that test them
+    const uint32_t rowOff = uint32_t(lane) * ngmax;
+    const TargetRec<T> myRec{Particle4<T>{xi, yi, zi, radiusSq}, rowOff, rowOff + ngmax,
+                             uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)) | (uint32_t(lane) << 8), 0u};
+    uint32_t myWpos = rowOff;
 
     // the wave's 64 output rows as a raw buffer: stores at an offset past its end are dropped by the hardware's
     // range check, so a lane without a hit stores nowhere instead of branching around the store
@@ -173,11 +179,12 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
             int pc          = __builtin_popcountll(m);
             const bool mine = (m >> lane) & 1u;
             const unsigned myRank = laneRank(m);
+            TargetRec<T>* recs    = &sh.ctgt[w * kWave];
             if (mine)
             {
                 TargetRec<T> r = myRec;
-                r.count        = myCount;
-                sh.ctgt[w][myRank] = r;
+                r.wpos         = myWpos;
+                recs[myRank]   = r;
             }
             waveSync();
             for (uint32_t cs = jb; cs < je; cs += kWave)
@@ -189,9 +196,15 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 uint32_t jj    = uint32_t(lane) - slot * nj;
                 bool jv        = slot < S;
                 uint32_t j     = cs + jj;
-                uint64_t below = (uint64_t(1) << (slot * nj)) - 1; // lanes of earlier slots (slot*nj < 64)
-                bool last      = jv && jj == nj - 1;
-                T xj = 0, yj = 0, zj = 0;
+                // lanes of this and later slots (slot * nj < 64): a hit's rank among its slot's hits
+                const uint64_t notBelow = ~((uint64_t(1) << (slot * nj)) - 1);
+                const uint32_t nbLo     = uint32_t(notBelow);
+                const uint32_t nbHi     = uint32_t(notBelow >> 32);
+                // the slot's last lane advances the count of the slot's target: the write condition last & (k < pc)
+                // is the single compare k < pcIfLast
+                const int pcIfLast = (jv && jj == nj - 1) ? pc : 0;
+                // lanes past the last slot test a NaN particle, which fails every distance compare
+                T xj = std::numeric_limits<T>::quiet_NaN(), yj = xj, zj = xj;
                 if (jv)
                 {
                     xj = x[j];
@@ -202,7 +215,10 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 const bool selfChunk = cs < uint32_t(first) + uint32_t(kWave) && cs + nj > uint32_t(first);
                 const uint32_t jRel  = j - uint32_t(first);
 
-                auto test = [&](const TargetRec<T>& rec, bool tv, auto checkSelf) -> bool {
+                // the pair test of particle j against a target record: distance (and, in chunks that overlap the
+                // wave's own targets, j != target); each condition is a single compare, so its ballot is the
+                // compare's own lane mask
+                auto close = [&](const TargetRec<T>& rec) -> bool {
                     bool fx = false, fy = false, fz = false;
                     if constexpr (AnyPbc)
                     {
@@ -210,50 +226,63 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                         fy = rec.meta & 2;
                         fz = rec.meta & 4;
                     }
-                    T xx     = fold(xj - rec.p.x, fx, 0);
-                    T yy     = fold(yj - rec.p.y, fy, 1);
-                    T zz     = fold(zj - rec.p.z, fz, 2);
-                    bool hit = tv & (xx * xx + yy * yy + zz * zz < rec.p.w);
-                    if constexpr (decltype(checkSelf)::value) { hit = hit & (jRel != (rec.meta >> 8)); }
-                    return hit;
+                    T xx = fold(xj - rec.p.x, fx, 0);
+                    T yy = fold(yj - rec.p.y, fy, 1);
+                    T zz = fold(zj - rec.p.z, fz, 2);
+                    return xx * xx + yy * yy + zz * zz < rec.p.w;
+                };
+                // slot position of a hit in its target's row: wpos + rank among the slot's hits
+                auto position = [&](uint64_t H, uint32_t wpos) -> uint32_t {
+                    return __builtin_amdgcn_mbcnt_hi(uint32_t(H >> 32) & nbHi,
+                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(H) & nbLo, wpos));
                 };
                 auto iterate = [&](auto checkSelf) {
-                    for (int k0 = 0; k0 < pc; k0 += 2 * int(S))
+                    // records of the slot's two targets of an iteration, walked by pointer: k < pc is
+                    // &recs[k] < recs + pc, and the count write condition is &recs[k] < writeEnd
+                    TargetRec<T>* rA               = recs + slot; // index < 128: pc <= 64 and slot < S <= 64
+                    const TargetRec<T>* const end  = recs + pc;
+                    const TargetRec<T>* writeEnd   = recs + pcIfLast;
+                    for (int k0 = 0; k0 < pc; k0 += 2 * int(S), rA += 2 * S)
                     {
-                        int kA   = k0 + int(slot); // < 128: pc <= 64 and slot < S <= 64
-                        int kB   = kA + int(S);
-                        bool tvA = jv & (kA < pc);
-                        bool tvB = jv & (kB < pc);
-                        const TargetRec<T> recA = sh.ctgt[w][kA < kWave ? kA : kWave - 1];
-                        const TargetRec<T> recB = sh.ctgt[w][kB < kWave ? kB : kWave - 1];
-                        bool hitA     = test(recA, tvA, checkSelf);
-                        bool hitB     = test(recB, tvB, checkSelf);
-                        uint64_t HA   = ballot(hitA);
-                        uint64_t HB   = ballot(hitB);
-                        uint32_t posA = recA.count + laneRank(HA & ~below);
-                        uint32_t posB = recB.count + laneRank(HB & ~below);
-                        uint32_t offA = (hitA & (posA < ngmax)) ? (recA.rowOff + posA) * 4u : rowsBytes;
-                        uint32_t offB = (hitB & (posB < ngmax)) ? (recB.rowOff + posB) * 4u : rowsBytes;
+                        TargetRec<T>* rB        = rA + S;
+                        const bool inA          = rA < end;
+                        const bool inB          = rB < end;
+                        const TargetRec<T> recA = *rA;
+                        const TargetRec<T> recB = *rB;
+                        bool cA = close(recA), cB = close(recB);
+                        uint64_t HA = ballot(cA) & ballot(inA);
+                        uint64_t HB = ballot(cB) & ballot(inB);
+                        if constexpr (decltype(checkSelf)::value)
+                        {
+                            const bool oA = jRel != (recA.meta >> 8), oB = jRel != (recB.meta >> 8);
+                            HA &= ballot(oA);
+                            HB &= ballot(oB);
+                            cA = cA & oA;
+                            cB = cB & oB;
+                        }
+                        const bool hitA     = inA & cA;
+                        const bool hitB     = inB & cB;
+                        const uint32_t posA = position(HA, recA.wpos);
+                        const uint32_t posB = position(HB, recB.wpos);
+                        const uint32_t offA = (hitA & (posA < recA.limit)) ? posA * 4u : rowsBytes;
+                        const uint32_t offB = (hitB & (posB < recB.limit)) ? posB * 4u : rowsBytes;
                         __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offA), 0, 0);
                         __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offB), 0, 0);
-                        uint32_t* cA = (last & tvA) ? &sh.ctgt[w][kA & (kWave - 1)].count : &sh.dummy[w][lane];
-                        uint32_t* cB = (last & tvB) ? &sh.ctgt[w][kB & (kWave - 1)].count : &sh.dummy[w][lane];
-                        *cA          = posA + (hitA ? 1u : 0u);
-                        *cB          = posB + (hitB ? 1u : 0u);
+                        if (rA < writeEnd) rA->wpos = posA + (hitA ? 1u : 0u);
+                        if (rB < writeEnd) rB->wpos = posB + (hitB ? 1u : 0u);
                     }
                 };
                 if (selfChunk) iterate(std::true_type{});
                 else iterate(std::false_type{});
                 waveSync();
             }
-            if (mine) myCount = sh.ctgt[w][myRank].count;
+            if (mine) myWpos = recs[myRank].wpos;
             waveSync();
         }
         listCount = 0;
     };
 
-    auto appendLeaf = [&](int32_t node, uint64_t m) {
-        int32_t leafIdx = uniform(internalToLeaf[node]);
+    auto appendLeaf = [&](int32_t leafIdx, uint64_t m) {
         if (lane == 0)
         {
             sh.listLeaf[w][listCount] = leafIdx;
@@ -265,7 +294,8 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     };
 
     // evaluate the 8 children of a node for the lanes in `pm` and push them as a stack frame; the 48 node values
-    // are loaded up front (contiguous, wave-uniform) and the tests are branch-free
+    // and the children's first-child / leaf indices are loaded up front (contiguous, wave-uniform) and the tests
+    // are branch-free
     auto pushFrame = [&](int depth, int32_t co, uint64_t pm) {
         const bool eligible = (pm >> lane) & 1u;
         const T* c          = centers + size_t(co) * 3;
@@ -288,7 +318,9 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         {
 #pragma unroll
             for (int k = 0; k < 8; ++k)
+            {
                 sh.mask[w][depth][k] = mk[k];
+            }
             sh.co[w][depth]   = co;
             sh.next[w][depth] = 0;
         }
@@ -301,7 +333,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     if (rootMask != 0)
     {
         int32_t rootChild = uniform(childOffsets[0]);
-        if (rootChild == 0) { appendLeaf(0, rootMask); }
+        if (rootChild == 0) { appendLeaf(uniform(internalToLeaf[0]), rootMask); }
         else
         {
             int depth = 0;
@@ -327,7 +359,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 waveSync();
                 if (node <= 0 || node >= numNodes) break; // malformed tree guard
                 int32_t cco = uniform(childOffsets[node]);
-                if (cco == 0) { appendLeaf(node, m); }
+                if (cco == 0) { appendLeaf(uniform(internalToLeaf[node]), m); }
                 else
                 {
                     if (depth + 1 >= kMaxDepth || cco <= node || cco + 8 > numNodes) break;
@@ -339,7 +371,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         drain();
     }
 
-    if (valid) neighborsCount[i - firstId] = myCount;
+    if (valid) neighborsCount[i - firstId] = myWpos - rowOff;
 }
 
 template<class T>
