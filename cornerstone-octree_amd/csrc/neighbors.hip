@@ -297,7 +297,6 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     // and the children's first-child / leaf indices are loaded up front (contiguous, wave-uniform) and the tests
     // are branch-free
     auto pushFrame = [&](int depth, int32_t co, uint64_t pm) {
-        const bool eligible = (pm >> lane) & 1u;
         const T* c          = centers + size_t(co) * 3;
         const T* s          = sizes + size_t(co) * 3;
         T cv[24], sv[24];
@@ -308,11 +307,13 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
             sv[k] = s[k];
         }
         uint64_t mk[8];
+        // the ballot of the bare compare is the compare's own lane mask; the parent's lanes are applied as a
+        // scalar AND
 #pragma unroll
         for (int k = 0; k < 8; ++k)
         {
             T d2  = nodeDist2(cv[3 * k], cv[3 * k + 1], cv[3 * k + 2], sv[3 * k], sv[3 * k + 1], sv[3 * k + 2]);
-            mk[k] = ballot(eligible & (d2 < cellR2));
+            mk[k] = ballot(d2 < cellR2) & pm;
         }
         if (lane == 0)
         {
