@@ -411,7 +411,8 @@ def main():
             "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
             "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         },
-        "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()},
+        "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()
+                   if c > 0},
     }
     if not args.no_cpu_baseline:
         t1 = time.time()
