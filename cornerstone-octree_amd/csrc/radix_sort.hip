@@ -8,10 +8,11 @@
 // keys; then each 8-bit digit pass is a single kernel that
 //   1. takes a tile id from an atomic counter (ids follow launch order, so every tile waits only on tiles that
 //      have already started: the look-back always terminates),
-//   2. ranks its 4096 keys stably with wave64 match-by-ballot (8 ballots per key) and per-wave LDS digit counters,
+//   2. ranks its 5888 keys stably with wave64 match-by-ballot (8 ballots per key) and per-wave LDS digit counters,
 //   3. publishes its per-digit counts and resolves its global digit offsets by decoupled look-back
 //      (one thread per digit, status word = 2-bit flag | 30-bit count),
-//   4. reorders the tile in LDS and writes each digit run contiguously.
+//   4. reorders the tile in LDS (keys, then values in the same storage, so a block holds 8 bytes per key and three
+//      blocks fit in a CU's LDS) and writes each digit run contiguously.
 // The last pass optionally writes the permutation and gathers x,y,z,h through it. Gathering 4 separate arrays at
 // random positions fetches 4 memory lines per particle; when the input is far from key order (more than n/8
 // descents, counted by the histogram kernel), x,y,z,h are first packed into one 4-value record per particle so that
@@ -29,8 +30,8 @@ namespace hipcstone
 namespace sortcfg
 {
 constexpr int kThreads   = 256;
-constexpr int kItems     = 16;
-constexpr int kTile      = kThreads * kItems; // 4096 keys per tile
+constexpr int kItems     = 23;
+constexpr int kTile      = kThreads * kItems; // 5888 keys per tile
 constexpr int kRadixBits = 8;
 constexpr int kRadix     = 1 << kRadixBits;
 constexpr int kWaves     = kThreads / kWave;
@@ -214,8 +215,12 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
                                                                uint32_t* __restrict__ tileCounter,
                                                                uint32_t* __restrict__ status, GatherArgs<GT> g)
 {
-    __shared__ K shKeys[kTile];
-    __shared__ uint32_t shVals[ValueMode != 0 ? kTile : 1];
+    // the tile is reordered through LDS twice, keys first, then values in the same storage
+    __shared__ union
+    {
+        K keys[kTile];
+        uint32_t vals[kTile];
+    } shTile;
     __shared__ uint32_t shWaveHist[kWaves][kRadix];
     __shared__ uint32_t shLocalStart[kRadix];
     __shared__ uint32_t shGlobalBase[kRadix];
@@ -310,23 +315,40 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
     uint32_t exclusive = 0;
     if (tile > 0)
     {
+        // status words of other tiles live in other XCDs' caches, so each read is a long round trip: read a window
+        // of kLookBack predecessors per trip and consume them nearest first, up to the first unpublished one
+        constexpr int kLookBack = 4;
         int64_t j = int64_t(tile) - 1;
         uint32_t spins = 0;
-        while (j >= 0)
+        bool done = false;
+        while (!done && j >= 0)
         {
-            uint32_t s = atomicLoadRelaxed(status + size_t(j) * kRadix + tid);
-            uint32_t flag = s & ~kValueMask;
-            if (flag == 0)
+            uint32_t w[kLookBack];
+#pragma unroll
+            for (int q = 0; q < kLookBack; ++q)
+                w[q] = j - q >= 0 ? atomicLoadRelaxed(status + size_t(j - q) * kRadix + tid) : kFlagPrefix;
+            bool stalled = false;
+#pragma unroll
+            for (int q = 0; q < kLookBack; ++q)
             {
-                // predecessor has not published yet; every tile publishes before it waits, so this ends. The cap
-                // only guards against a corrupted workspace.
+                if (done || stalled) continue;
+                uint32_t flag = w[q] & ~kValueMask;
+                if (flag == 0)
+                {
+                    stalled = true;
+                    continue;
+                }
+                exclusive += w[q] & kValueMask;
+                if (flag == kFlagPrefix) done = true;
+                else --j;
+            }
+            if (stalled)
+            {
+                // a predecessor has not published yet; every tile publishes before it waits, so this ends. The
+                // cap only guards against a corrupted workspace.
                 if (++spins > (1u << 22)) break;
                 __builtin_amdgcn_s_sleep(1);
-                continue;
             }
-            exclusive += s & kValueMask;
-            if (flag == kFlagPrefix) break;
-            --j;
         }
         atomicStoreRelaxed(myStatus + tid, kFlagPrefix | (exclusive + tileCount));
     }
@@ -339,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
     shGlobalBase[tid]     = digitOffsets[tid] + exclusive - localStart;
     __syncthreads();
 
-    // ---- scatter into LDS in sorted order ----
+    // ---- scatter the keys into LDS in sorted order (ranks[k] becomes the key's slot in the tile) ----
     uint32_t validInTile = n > tileBase ? uint32_t(min(size_t(kTile), n - tileBase)) : 0u;
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
@@ -347,27 +369,48 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
         size_t i = waveBase + size_t(k) * kWave + lane;
         if (i < n)
         {
-            unsigned d   = unsigned(keys[k] >> shift) & digitMask;
-            uint32_t pos = shLocalStart[d] + shWaveHist[wave][d] + ranks[k];
-            shKeys[pos]  = keys[k];
-            if constexpr (ValueMode != 0) shVals[pos] = vals[k];
+            unsigned d        = unsigned(keys[k] >> shift) & digitMask;
+            ranks[k]          = shLocalStart[d] + shWaveHist[wave][d] + ranks[k];
+            shTile.keys[ranks[k]] = keys[k];
         }
     }
     __syncthreads();
 
-    // ---- write out: consecutive LDS slots of one digit go to consecutive global slots ----
+    // ---- write out: consecutive LDS slots of one digit go to consecutive global slots; slot i = tid + m * 256 ----
+    uint32_t dsts[kItems];
+#pragma unroll
+    for (int m = 0; m < kItems; ++m)
+    {
+        uint32_t i = uint32_t(tid) + uint32_t(m) * kThreads;
+        if (i < validInTile)
+        {
+            K key        = shTile.keys[i];
+            unsigned d   = unsigned(key >> shift) & digitMask;
+            dsts[m]      = shGlobalBase[d] + i; // modulo 2^32: the base alone may wrap below zero
+            keysOut[dsts[m]] = key;
+        }
+    }
+    if constexpr (ValueMode == 0) return;
+    __syncthreads(); // every key is read out before the values reuse the storage
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        size_t i = waveBase + size_t(k) * kWave + lane;
+        if (i < n) shTile.vals[ranks[k]] = vals[k];
+    }
+    __syncthreads();
+
     bool usePacked = false;
     if constexpr (GatherOn) usePacked = g.packed && *g.descents > g.threshold;
-    for (uint32_t i = tid; i < validInTile; i += kThreads)
+#pragma unroll
+    for (int m = 0; m < kItems; ++m)
     {
-        K key        = shKeys[i];
-        unsigned d   = unsigned(key >> shift) & digitMask;
-        uint32_t dst = shGlobalBase[d] + i; // modulo 2^32: the base alone may wrap below zero
-        keysOut[dst] = key;
-        if constexpr (ValueMode != 0)
+        uint32_t i = uint32_t(tid) + uint32_t(m) * kThreads;
+        if (i < validInTile)
         {
-            uint32_t v     = shVals[i];
-            valuesOut[dst] = v;
+            const uint32_t dst = dsts[m];
+            uint32_t v         = shTile.vals[i];
+            valuesOut[dst]     = v;
             if constexpr (GatherOn)
             {
                 if (usePacked)
