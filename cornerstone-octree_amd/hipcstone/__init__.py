@@ -398,6 +398,18 @@ def find_halos(prefixes, child_offsets, parents, centers, sizes, leaves, search_
              C.c_int32(num_nodes), _p(flags), _p(workspace), C.c_size_t(wsb)), "find_halos")
 
 
+def ijloop_example(x, y, z, h, m, first, last, box, child_offsets, parents, internal_to_leaf, layout, centers, sizes,
+                   num_nodes, ngmax, rho, count, dmin, dmax, fx, ext=1.0, stream=None):
+    """The fixed example interaction through the ijloop API (hipcstone_ijloop.hpp; reference
+    traversal/ijloop/gpu_alwaystraverse.cuh:174-204): outputs are body-indexed device arrays."""
+    T = np.dtype(box_dtype(box))
+    f = _fn(f"hipcstone_ijloop_example_{_RS[T]}")
+    _check(f(_stream(stream), _p(child_offsets), _p(parents), _p(internal_to_leaf), _p(layout), _p(centers), _p(sizes),
+             C.c_int32(num_nodes), C.byref(box), C.c_uint32(first), C.c_uint32(last), _p(x), _p(y), _p(z), _p(h),
+             _p(m), C.c_uint32(ngmax), C.c_float(ext), _p(rho), _p(count), _p(dmin), _p(dmax), _p(fx)),
+           "ijloop_example")
+
+
 def _dtype_of(a, default=None):
     if isinstance(a, DeviceArray):
         return np.dtype(a.dtype)

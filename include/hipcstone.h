@@ -369,6 +369,23 @@ HIPCSTONE_DECL_GATHER_LEAVES(u64, uint64_t)
 HIPCSTONE_DECL_NEIGHBORS(f32, float)
 HIPCSTONE_DECL_NEIGHBORS(f64, double)
 
+/* ---- ijloop (the step after the neighbor search). HIP C++ callers include hipcstone_ijloop.hpp, the counterpart of
+ * cstone::ijloop's GpuAlwaysTraverseNeighborhoodBuilder (traversal/ijloop/gpu_alwaystraverse.cuh:174-204, interface
+ * ijloop.hpp:29-146), and run their own interaction functors. This C entry runs one fixed interaction through it
+ * (ijloop_examples.hip): per body i in [first, last), over i itself and its first ngmax neighbors in ascending order,
+ * with w = (1 - d2 / ((4 h_i) h_i))^3: rho = sum m_j w / ((h_i h_i) h_i), count = 1 + neighbors used,
+ * dmin / dmax = min / max of d2 over the neighbors, fx = sum of m_j (x_i - x_j) (minimum image for bodies whose 2h box
+ * leaves a periodic domain). It equals the reference CPU ijloop (cpu_alwaystraverse.hpp:90-113) running the same
+ * interaction. ---- */
+#define HIPCSTONE_DECL_IJLOOP_EXAMPLE(RS, T)                                                                           \
+    hipcstone_status hipcstone_ijloop_example_##RS(                                                                    \
+        hipcstone_stream s, const int32_t* child_offsets, const int32_t* parents, const int32_t* internal_to_leaf,    \
+        const uint32_t* layout, const T* centers, const T* sizes, int32_t num_nodes, const hipcstone_box_##RS* box,   \
+        uint32_t first, uint32_t last, const T* x, const T* y, const T* z, const T* h, const T* m, uint32_t ngmax,    \
+        float ext, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx);
+HIPCSTONE_DECL_IJLOOP_EXAMPLE(f32, float)
+HIPCSTONE_DECL_IJLOOP_EXAMPLE(f64, double)
+
 /* ---- plain pipeline: one sync step of the single-domain tree, host orchestration of the entries above.
  * Replaces the sequence Domain<KeyType,T,Gpu>::sync at numRanks=1 performs on its global tree
  * (domain/domain.hpp:169-218 -> domain/assignment.hpp:92-144) followed by octreeProperties() + findNeighbors

@@ -28,6 +28,7 @@
  *   gather    brute-force element copy and range enumeration (the reference gatherRanges is GPU-only, not built)
  */
 
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -74,24 +75,28 @@
 #define FLOOR floorf
 #define RINT rintf
 #define FABS fabsf
+#define REAL_MAX FLT_MAX
 #include "oracle_real.inc"
 #undef REAL
 #undef RS
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef REAL_MAX
 
 #define REAL double
 #define RS f64
 #define FLOOR floor
 #define RINT rint
 #define FABS fabs
+#define REAL_MAX DBL_MAX
 #include "oracle_real.inc"
 #undef REAL
 #undef RS
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef REAL_MAX
 
 /* ---- key x coordinate templates ---- */
 #define KEY uint32_t

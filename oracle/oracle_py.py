@@ -216,6 +216,19 @@ class CpuLib:
           _ptr(nb), _ptr(nc))
         return nb[: n * ngmax].reshape(n, ngmax) if ngmax else nb[:0].reshape(n, 0), nc[:n]
 
+    def ijloop_example(self, x, y, z, h, m, first, last, box, tree, layout, centers, sizes, ngmax, ext=1.0):
+        """The example ijloop interaction (rho, count, dmin, dmax, fx), body-indexed; entries outside [first, last)
+        stay zero."""
+        T = x.dtype
+        n = x.size
+        out = [np.zeros(n, T), np.zeros(n, np.uint32), np.zeros(n, T), np.zeros(n, T), np.zeros(n, T)]
+        f = self._f(f"ijloop_example_{_RS[T]}")
+        f(_ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(np.ascontiguousarray(m, dtype=T)), C.c_uint32(first),
+          C.c_uint32(last), C.byref(box), _ptr(tree["childOffsets"]), _ptr(tree["parents"]),
+          _ptr(tree["internalToLeaf"]), _ptr(layout), _ptr(np.ascontiguousarray(centers)),
+          _ptr(np.ascontiguousarray(sizes)), C.c_float(ext), C.c_uint(ngmax), *[_ptr(a) for a in out])
+        return out
+
     def find_neighbors_subset(self, x, y, z, h, ids, box, tree, layout, centers, sizes, ngmax, ext=1.0):
         """Per-particle search for a list of indices (oracle only)."""
         assert self.kind == "oracle"

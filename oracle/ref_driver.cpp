@@ -21,6 +21,7 @@
 #include "cstone/primitives/gather.hpp"
 #include "cstone/sfc/sfc.hpp"
 #include "cstone/traversal/collisions.hpp"
+#include "cstone/traversal/ijloop/cpu_alwaystraverse.hpp"
 #include "cstone/tree/csarray.hpp"
 #include "cstone/tree/octree.hpp"
 
@@ -131,6 +132,49 @@ void findNeighborsImpl(const T* x, const T* y, const T* z, const T* h, uint32_t 
     v.sizes           = v3(sizes);
     v.searchExtFactor = ext;
     findNeighbors(x, y, z, h, first, last, toBox(box), v, ngmax, nb, nc);
+}
+
+// The reference's CPU ijloop (CpuAlwaysTraverseNeighborhoodBuilder) running the example interaction of
+// hipcstone_ijloop_example_*: w = (1 - d2 / r2)^3 with r2 = radiusSq-style (4 h_i) h_i, sums, count, min / max
+// reductions, an odd-symmetric marker, and a postamble dividing by (h_i h_i) h_i.
+template<class T>
+void ijloopExampleImpl(const T* x, const T* y, const T* z, const T* h, const T* m, uint32_t first, uint32_t last,
+                       const BoxC<T>* box, const int32_t* childOffsets, const int32_t* parents,
+                       const int32_t* internalToLeaf, const uint32_t* layout, const T* centers, const T* sizes,
+                       float ext, unsigned ngmax, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx)
+{
+    OctreeNsView<T, uint64_t> v{};
+    v.childOffsets    = childOffsets;
+    v.parents         = parents;
+    v.internalToLeaf  = internalToLeaf;
+    v.layout          = layout;
+    v.centers         = v3(centers);
+    v.sizes           = v3(sizes);
+    v.searchExtFactor = ext;
+    GroupView groups{first, last, 0, nullptr, nullptr};
+    auto interaction = [](auto const& iData, auto const& jData, Vec3<T> const& r, T d2)
+    {
+        const T hi     = std::get<2>(iData);
+        const T mj     = std::get<3>(jData);
+        const T r2     = T(4) * hi * hi;
+        const T q2     = d2 / r2;
+        const T u      = T(1) - q2;
+        const T w      = u * u * u;
+        const bool own = std::get<0>(iData) == std::get<0>(jData);
+        return std::make_tuple(mj * w, uint32_t(1),
+                               ijloop::reduction::min<T>{own ? std::numeric_limits<T>::max() : d2},
+                               ijloop::reduction::max<T>{own ? std::numeric_limits<T>::lowest() : d2},
+                               ijloop::symmetric::odd<T>{mj * r[0]});
+    };
+    auto postamble = [](auto const& iData, std::tuple<T, uint32_t, T, T, T> const& res)
+    {
+        const T hi = std::get<2>(iData);
+        return std::make_tuple(std::get<0>(res) / (hi * hi * hi), std::get<1>(res), std::get<2>(res),
+                               std::get<3>(res), std::get<4>(res));
+    };
+    ijloop::CpuAlwaysTraverseNeighborhoodBuilder{ngmax}
+        .build(execution::Cpu{}, v, toBox(box), last, groups, x, y, z, h)
+        .ijLoop(std::make_tuple(m), std::make_tuple(rho, count, dmin, dmax, fx), interaction, postamble);
 }
 
 template<class K, class T>
@@ -561,6 +605,15 @@ extern "C"
                                        int32_t first, int32_t last, T scale, T* centers, T* sizes)                     \
     {                                                                                                                  \
         bboxImpl<T>(x, y, z, h, layout, first, last, scale, centers, sizes);                                           \
+    }                                                                                                                  \
+    void ref_ijloop_example_##TS(const T* x, const T* y, const T* z, const T* h, const T* m, uint32_t first,         \
+                                 uint32_t last, const BoxC<T>* box, const int32_t* childOffsets,                      \
+                                 const int32_t* parents, const int32_t* internalToLeaf, const uint32_t* layout,        \
+                                 const T* centers, const T* sizes, float ext, unsigned ngmax, T* rho,                  \
+                                 uint32_t* count, T* dmin, T* dmax, T* fx)                                             \
+    {                                                                                                                  \
+        ijloopExampleImpl<T>(x, y, z, h, m, first, last, box, childOffsets, parents, internalToLeaf, layout, centers,  \
+                             sizes, ext, ngmax, rho, count, dmin, dmax, fx);                                           \
     }
 
     REF_T(float, f32)
