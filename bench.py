@@ -264,6 +264,93 @@ def domain_bench(args):
     print(json.dumps(result), flush=True)
 
 
+def ijloop_bench(args):
+    """ijloop over the plain pipeline's tree: the same 16,777,216 fp32 Plummer particles (h from the analytic
+    density), Hilbert-64 keys, bucketSize 64, a converged tree; one step = one ijLoop of the example interaction
+    (hipcstone_ijloop.hpp: neighbor search into chunked scratch, then the per-body interaction fold over its first
+    ngmax = 150 neighbors: kernel sum, count, min / max distance, odd-symmetric x moment, postamble)."""
+    import hipcstone as hip
+    from hipcstone import inputs
+
+    n, K, W, ngmax = args.n, args.steps, args.warmup, args.ngmax
+    T = np.float32
+    a = 0.05
+    x, y, z, r = inputs.plummer(n, a, seed=2024, dtype=T)
+    h = inputs.plummer_h(r, n, a, 100.0, dtype=T)
+    m = (np.random.default_rng(3).random(n) + 0.5).astype(T)
+    box = hip.make_box(-1.0, 1.0, T)
+    pipe = hip.Pipeline(curve=hip.HILBERT, key_bits=64, real_bits=32, bucket_size=64, ngmax=0)
+    d = [hip.DeviceArray.from_numpy(v) for v in (x, y, z, h)]
+    pipe.sync(*d, n, box, full_build=True)
+    v = pipe.view()
+    perm = hip.download_ptr(v.perm, n, np.uint32)
+    dm = hip.DeviceArray.from_numpy(m[perm])
+    outs = [hip.DeviceArray(n, T), hip.DeviceArray(n, np.uint32), hip.DeviceArray(n, T), hip.DeviceArray(n, T),
+            hip.DeviceArray(n, T)]
+
+    def step():
+        hip.ijloop_example(v.x, v.y, v.z, v.h, dm, 0, n, box, v.child_offsets, v.parents, v.internal_to_leaf,
+                           v.layout, v.centers, v.sizes, v.num_nodes, ngmax, *outs)
+
+    for _ in range(W):
+        step()
+    hip.device_sync()
+    events = [hip.Event() for _ in range(K + 1)]
+    events[0].record()
+    for k in range(K):
+        step()
+        events[k + 1].record()
+    hip.device_sync()
+    per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
+    total_ms = float(per_step.sum())
+    cnt = outs[1].download()
+    result = {
+        "metric": "Mparticles/s ijLoop (neighbor search + interaction fold, 16M fp32)",
+        "value": round(n * K / (total_ms / 1e3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(total_ms / K, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": ("ijLoop of the example interaction over 16,777,216 fp32 Plummer particles (a=0.05), "
+                         "Hilbert-64 keys, bucketSize 64, converged tree, ngmax 150"),
+            "particles": n, "leaves": int(v.num_leaves), "mean_terms": round(float(cnt.mean()), 2),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
+                       "p99": round(float(np.percentile(per_step, 99)), 3)},
+        "cpu_baseline": None if args.no_cpu_baseline else ijloop_cpu_baseline(v, n, m[perm], ngmax),
+    }
+    print(json.dumps(result), flush=True)
+
+
+def ijloop_cpu_baseline(v, n, m, ngmax, stride=32):
+    """The reference's CPU ijloop (oracle/_ref, CpuAlwaysTraverseNeighborhood) with the same interaction over the
+    first n/stride bodies of the same sorted particles and tree, scaled to n."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_py import REF, make_box
+    import hipcstone as hip
+
+    if REF is None:
+        return None
+    T = np.float32
+    L, N = int(v.num_leaves), int(v.num_nodes)
+    arrs = {k: hip.download_ptr(getattr(v, k), n, T) for k in ("x", "y", "z", "h")}
+    tree = {"childOffsets": hip.download_ptr(v.child_offsets, N, np.int32),
+            "parents": hip.download_ptr(v.parents, max((N - 1) // 8, 1), np.int32),
+            "internalToLeaf": hip.download_ptr(v.internal_to_leaf, N, np.int32)}
+    layout = hip.download_ptr(v.layout, L + 1, np.uint32)
+    centers = hip.download_ptr(v.centers, 3 * N, T)
+    sizes = hip.download_ptr(v.sizes, 3 * N, T)
+    last = n // stride
+    t0 = time.time()
+    REF.ijloop_example(arrs["x"], arrs["y"], arrs["z"], arrs["h"], m, 0, last, make_box(-1.0, 1.0, T), tree, layout,
+                       centers, sizes, ngmax)
+    ms = (time.time() - t0) * 1e3
+    return {"value": round(last / (ms / 1e3) / 1e6, 3), "unit": "Mparticles/s", "cores": REF.omp_threads(),
+            "kind": "reference",
+            "sample": (f"the reference CPU ijloop (CpuAlwaysTraverseNeighborhood, oracle/_ref) with the same "
+                       f"interaction over bodies [0, {last}) of the same sorted particles and tree: {ms:.0f} ms")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,9 +360,9 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "halo", "domain"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "halo", "domain", "ijloop"], default="plain",
                     help="plain: the 16.8M neighbor pipeline (default); halo: the 134M rank-share chain; "
-                         "domain: one-rank Domain::sync + neighbors at 16.8M")
+                         "domain: one-rank Domain::sync + neighbors at 16.8M; ijloop: the ijLoop example at 16.8M")
     ap.add_argument("--n-halo", type=int, default=134217728)
     ap.add_argument("--halo-cpu-sample", type=int, default=8388608,
                     help="particles of the CPU baseline sample of the halo workload")
@@ -308,6 +395,8 @@ def main():
     if args.workload == "halo":
         halo_bench(args)
         return
+    if args.workload == "ijloop":
+        return ijloop_bench(args)
     if args.workload == "domain":
         domain_bench(args)
         return

@@ -90,7 +90,9 @@ struct NbShared
     TargetRec<T> ctgt[kNbWaves * kWave + kWave];
 };
 
-template<class T, bool AnyPbc>
+// SlotMajor: the wave's 64 rows are stored slot-major (neighbor k of the wave's lane l at k * 64 + l) instead of
+// row-major (l * ngmax + k), so that a consumer with one thread per target reads each slot with one coalesced load.
+template<class T, bool AnyPbc, bool SlotMajor>
 __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, const T* __restrict__ h,
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
@@ -149,15 +151,17 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     // per-target state: the lane of target t keeps its next row slot in a register (the count is wpos - rowOff, all
     // neighbors counted, only the first ngmax stored); while a leaf is drained the leaf's active targets are copied,
     // compacted, into LDS for the lanes that test them
-    const uint32_t rowOff = uint32_t(lane) * ngmax;
-    const TargetRec<T> myRec{Particle4<T>{xi, yi, zi, radiusSq}, rowOff, rowOff + ngmax,
+    constexpr uint32_t kSlotStep = SlotMajor ? uint32_t(kWave) : 1u; // element distance of consecutive slots
+    const uint32_t rowOff        = SlotMajor ? uint32_t(lane) : uint32_t(lane) * ngmax;
+    const TargetRec<T> myRec{Particle4<T>{xi, yi, zi, radiusSq}, rowOff, rowOff + ngmax * kSlotStep,
                              uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)) | (uint32_t(lane) << 8), 0u};
     uint32_t myWpos = rowOff;
 
     // the wave's 64 output rows as a raw buffer: stores at an offset past its end are dropped by the hardware's
     // range check, so a lane without a hit stores nowhere instead of branching around the store
     const uint32_t validRows = uint64_t(lastId) - first < uint64_t(kWave) ? uint32_t(uint64_t(lastId) - first) : kWave;
-    const uint32_t rowsBytes = ngmax ? validRows * ngmax * uint32_t(sizeof(uint32_t)) : 0u;
+    const uint32_t rowsBytes = ngmax ? (SlotMajor ? uint32_t(kWave) : validRows) * ngmax * uint32_t(sizeof(uint32_t))
+                                     : 0u;
     const __amdgpu_buffer_rsrc_t rows =
         __builtin_amdgcn_make_buffer_rsrc(ngmax ? static_cast<void*>(neighbors + rowBase * ngmax) : nullptr, 0,
                                           int(rowsBytes), kRawBufferFlags);
@@ -233,8 +237,17 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 };
                 // slot position of a hit in its target's row: wpos + rank among the slot's hits
                 auto position = [&](uint64_t H, uint32_t wpos) -> uint32_t {
-                    return __builtin_amdgcn_mbcnt_hi(uint32_t(H >> 32) & nbHi,
-                                                     __builtin_amdgcn_mbcnt_lo(uint32_t(H) & nbLo, wpos));
+                    if constexpr (SlotMajor)
+                    {
+                        uint32_t r = __builtin_amdgcn_mbcnt_hi(uint32_t(H >> 32) & nbHi,
+                                                               __builtin_amdgcn_mbcnt_lo(uint32_t(H) & nbLo, 0u));
+                        return wpos + (r << 6);
+                    }
+                    else
+                    {
+                        return __builtin_amdgcn_mbcnt_hi(uint32_t(H >> 32) & nbHi,
+                                                         __builtin_amdgcn_mbcnt_lo(uint32_t(H) & nbLo, wpos));
+                    }
                 };
                 auto iterate = [&](auto checkSelf) {
                     // records of the slot's two targets of an iteration, walked by pointer: k < pc is
@@ -268,8 +281,8 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                         const uint32_t offB = (hitB & (posB < recB.limit)) ? posB * 4u : rowsBytes;
                         __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offA), 0, 0);
                         __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offB), 0, 0);
-                        if (rA < writeEnd) rA->wpos = posA + (hitA ? 1u : 0u);
-                        if (rB < writeEnd) rB->wpos = posB + (hitB ? 1u : 0u);
+                        if (rA < writeEnd) rA->wpos = posA + (hitA ? kSlotStep : 0u);
+                        if (rB < writeEnd) rB->wpos = posB + (hitB ? kSlotStep : 0u);
                     }
                 };
                 if (selfChunk) iterate(std::true_type{});
@@ -372,7 +385,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         drain();
     }
 
-    if (valid) neighborsCount[i - firstId] = myWpos - rowOff;
+    if (valid) neighborsCount[i - firstId] = SlotMajor ? (myWpos - rowOff) >> 6 : myWpos - rowOff;
 }
 
 template<class T>
@@ -381,7 +394,7 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
                                       const int32_t* childOffsets, const int32_t* parents,
                                       const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,
                                       const T* sizes, int32_t numNodes, float ext, uint32_t ngmax,
-                                      uint32_t* neighbors, uint32_t* neighborsCount)
+                                      uint32_t* neighbors, uint32_t* neighborsCount, bool slotMajor)
 {
     if (lastId <= firstId) return HIPCSTONE_OK;
     if (!x || !y || !z || !h || !box || !childOffsets || !parents || !internalToLeaf || !layout || !centers ||
@@ -396,14 +409,15 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
     blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
     bool anyPbc     = box->bnd[0] == HIPCSTONE_PERIODIC || box->bnd[1] == HIPCSTONE_PERIODIC ||
                   box->bnd[2] == HIPCSTONE_PERIODIC;
-    if (anyPbc)
-        findNeighborsKernel<T, true><<<blocks, kNbThreads, 0, stream(s)>>>(
+    auto launch = [&](auto pbc, auto slots) {
+        findNeighborsKernel<T, decltype(pbc)::value, decltype(slots)::value><<<blocks, kNbThreads, 0, stream(s)>>>(
             x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
             ngmax, neighbors, neighborsCount, blocks);
-    else
-        findNeighborsKernel<T, false><<<blocks, kNbThreads, 0, stream(s)>>>(
-            x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
-            ngmax, neighbors, neighborsCount, blocks);
+    };
+    if (anyPbc && slotMajor) launch(std::true_type{}, std::true_type{});
+    else if (anyPbc) launch(std::true_type{}, std::false_type{});
+    else if (slotMajor) launch(std::false_type{}, std::true_type{});
+    else launch(std::false_type{}, std::false_type{});
     HC_CHECK_LAUNCH("findNeighborsKernel");
     return HIPCSTONE_OK;
 }
@@ -423,7 +437,16 @@ extern "C"
         float ext, uint32_t ngmax, uint32_t* neighbors, uint32_t* neighbors_count)                                     \
     {                                                                                                                  \
         return findNeighbors<T>(s, x, y, z, h, first_id, last_id, box, child_offsets, parents, internal_to_leaf,      \
-                                layout, centers, sizes, num_nodes, ext, ngmax, neighbors, neighbors_count);            \
+                                layout, centers, sizes, num_nodes, ext, ngmax, neighbors, neighbors_count, false);     \
+    }                                                                                                                  \
+    hipcstone_status hipcstone_find_neighbors_slot_major_##RS(                                                         \
+        hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, uint32_t first_id, uint32_t last_id,      \
+        const hipcstone_box_##RS* box, const int32_t* child_offsets, const int32_t* parents,                          \
+        const int32_t* internal_to_leaf, const uint32_t* layout, const T* centers, const T* sizes, int32_t num_nodes,  \
+        float ext, uint32_t ngmax, uint32_t* neighbors, uint32_t* neighbors_count)                                     \
+    {                                                                                                                  \
+        return findNeighbors<T>(s, x, y, z, h, first_id, last_id, box, child_offsets, parents, internal_to_leaf,      \
+                                layout, centers, sizes, num_nodes, ext, ngmax, neighbors, neighbors_count, true);      \
     }
 
 HIPCSTONE_DEF_NEIGHBORS(f32, float)

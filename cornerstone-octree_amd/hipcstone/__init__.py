@@ -493,10 +493,11 @@ def gather_leaf_ranges(src_layout, dst_layout, num_leaves, srcs, dsts, dtype=Non
 
 
 def find_neighbors(x, y, z, h, first, last, box, child_offsets, parents, internal_to_leaf, layout, centers, sizes,
-                   num_nodes, ngmax, neighbors, counts, ext=1.0, stream=None):
-    """CPU-semantics radius search on the device (findneighbors.hpp:156-177)."""
+                   num_nodes, ngmax, neighbors, counts, ext=1.0, stream=None, slot_major=False):
+    """CPU-semantics radius search on the device (findneighbors.hpp:156-177). slot_major: rows of each group of 64
+    bodies stored slot-major (neighbor k of relative body r at ((r // 64) * ngmax + k) * 64 + r % 64)."""
     T = x.dtype if isinstance(x, DeviceArray) else np.dtype(box_dtype(box))
-    f = _fn(f"hipcstone_find_neighbors_{_RS[np.dtype(T)]}")
+    f = _fn(f"hipcstone_find_neighbors_{'slot_major_' if slot_major else ''}{_RS[np.dtype(T)]}")
     _check(f(_stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_uint32(first), C.c_uint32(last), C.byref(box),
              _p(child_offsets), _p(parents), _p(internal_to_leaf), _p(layout), _p(centers), _p(sizes),
              C.c_int32(num_nodes), C.c_float(ext), C.c_uint32(ngmax), _p(neighbors), _p(counts)), "find_neighbors")

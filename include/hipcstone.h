@@ -369,6 +369,21 @@ HIPCSTONE_DECL_GATHER_LEAVES(u64, uint64_t)
 HIPCSTONE_DECL_NEIGHBORS(f32, float)
 HIPCSTONE_DECL_NEIGHBORS(f64, double)
 
+/* ---- the same search with the neighbor rows stored slot-major per group of 64 bodies: for body i, r = i - first_id,
+ * neighbor k is at neighbors[((r / 64) * ngmax + k) * 64 + r % 64]; neighbors must hold ceil((last_id - first_id) /
+ * 64) * 64 * ngmax elements. No reference counterpart: it is the scratch layout of the ijloop consumer
+ * (hipcstone_ijloop.hpp), whose one-thread-per-body fold reads one slot of 64 bodies per coalesced load, like the
+ * per-warp column layout of the reference's GpuAlwaysTraverseNeighborhood (gpu_alwaystraverse.cuh:47-75). Counts
+ * are exact up to 2^26 neighbors per body. ---- */
+#define HIPCSTONE_DECL_NEIGHBORS_SLOT_MAJOR(RS, T)                                                                     \
+    hipcstone_status hipcstone_find_neighbors_slot_major_##RS(                                                         \
+        hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, uint32_t first_id, uint32_t last_id,      \
+        const hipcstone_box_##RS* box, const int32_t* child_offsets, const int32_t* parents,                          \
+        const int32_t* internal_to_leaf, const uint32_t* layout, const T* centers, const T* sizes, int32_t num_nodes,  \
+        float ext, uint32_t ngmax, uint32_t* neighbors, uint32_t* neighbors_count);
+HIPCSTONE_DECL_NEIGHBORS_SLOT_MAJOR(f32, float)
+HIPCSTONE_DECL_NEIGHBORS_SLOT_MAJOR(f64, double)
+
 /* ---- ijloop (the step after the neighbor search). HIP C++ callers include hipcstone_ijloop.hpp, the counterpart of
  * cstone::ijloop's GpuAlwaysTraverseNeighborhoodBuilder (traversal/ijloop/gpu_alwaystraverse.cuh:174-204, interface
  * ijloop.hpp:29-146), and run their own interaction functors. This C entry runs one fixed interaction through it

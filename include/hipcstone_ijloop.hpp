@@ -19,8 +19,10 @@
  *
  * The neighbor search runs in the library (libhipcstone.so); the interaction kernel is instantiated here, in the
  * caller's translation unit, for the caller's interaction and postamble (device-callable functors or lambdas).
- * Bodies are processed in chunks so that the neighbor scratch stays below 2 GiB. h must be an array (the reference
- * also accepts a constant). Errors are thrown as std::runtime_error carrying hipcstone_last_error().
+ * Bodies are processed in chunks so that the neighbor scratch stays below 2 GiB; the scratch holds each group of 64
+ * bodies' lists slot-major (hipcstone_find_neighbors_slot_major_*), so the fold's per-slot index loads coalesce.
+ * h must be an array (the reference also accepts a constant). Errors are thrown as std::runtime_error carrying
+ * hipcstone_last_error().
  */
 #pragma once
 
@@ -265,11 +267,13 @@ __global__ __launch_bounds__(256) void ijLoopKernel(Box<T> box, const T* __restr
         if constexpr (AnyPbc) usePbc = requiresPbc(box, pi, std::get<2>(iData));
         const LocalIndex row = i - chunkFirst;
         const unsigned nbs   = counts[row] < ngmax ? counts[row] : ngmax;
+        // slot-major scratch: neighbor nb of row r at ((r / 64) * ngmax + nb) * 64 + r % 64
+        const LocalIndex* slots = neighbors + size_t(row >> 6) * ngmax * 64 + (row & 63u);
         auto result          = interaction(iData, iData, Vec3<T>{{T(0), T(0), T(0)}}, T(0));
         constexpr auto idx   = std::make_index_sequence<std::tuple_size_v<decltype(result)>>{};
         for (unsigned nb = 0; nb < nbs; ++nb)
         {
-            const LocalIndex j = neighbors[size_t(row) * ngmax + nb];
+            const LocalIndex j = slots[size_t(nb) * 64];
             const auto jData   = loadParticleData(x, y, z, h, input, j, std::index_sequence_for<In...>{});
             Vec3<T> r;
             T d2;
@@ -318,7 +322,8 @@ inline hipcstone_status findNeighbors(hipStream_t s, const float* x, const float
                                       const void* sizes, int32_t numNodes, float ext, unsigned ngmax,
                                       LocalIndex* nb, LocalIndex* nc)
 {
-    return hipcstone_find_neighbors_f32(reinterpret_cast<hipcstone_stream>(s), x, y, z, h, first, last, &box,
+    return hipcstone_find_neighbors_slot_major_f32(reinterpret_cast<hipcstone_stream>(s), x, y, z, h, first, last,
+                                                   &box,
                                         childOffsets, parents, internalToLeaf, layout,
                                         static_cast<const float*>(centers), static_cast<const float*>(sizes),
                                         numNodes, ext, ngmax, nb, nc);
@@ -331,7 +336,8 @@ inline hipcstone_status findNeighbors(hipStream_t s, const double* x, const doub
                                       const void* sizes, int32_t numNodes, float ext, unsigned ngmax,
                                       LocalIndex* nb, LocalIndex* nc)
 {
-    return hipcstone_find_neighbors_f64(reinterpret_cast<hipcstone_stream>(s), x, y, z, h, first, last, &box,
+    return hipcstone_find_neighbors_slot_major_f64(reinterpret_cast<hipcstone_stream>(s), x, y, z, h, first, last,
+                                                   &box,
                                         childOffsets, parents, internalToLeaf, layout,
                                         static_cast<const double*>(centers), static_cast<const double*>(sizes),
                                         numNodes, ext, ngmax, nb, nc);
@@ -352,8 +358,9 @@ public:
         : stream_(stream), tree_(tree), box_(box), groups_(groups), x_(x), y_(y), z_(z), h_(h), ngmax_(ngmax)
     {
         LocalIndex bodies = groups.lastBody > groups.firstBody ? groups.lastBody - groups.firstBody : 0;
-        chunk_            = LocalIndex(std::min<std::size_t>(
-            std::max<LocalIndex>(bodies, 1), std::max<std::size_t>(kScratchElements / std::max(ngmax, 1u), 1)));
+        std::size_t fit   = std::max<std::size_t>(kScratchElements / std::max(ngmax, 1u) / 64 * 64, 64);
+        chunk_            = LocalIndex(std::min<std::size_t>((std::size_t(std::max<LocalIndex>(bodies, 1)) + 63) / 64 * 64,
+                                                             fit));
         void* p           = nullptr;
         detail::checkHip(hipMalloc(&p, std::max<std::size_t>(std::size_t(chunk_) * ngmax, 1) * sizeof(LocalIndex)),
                          "ijloop: neighbor scratch");

@@ -269,6 +269,30 @@ def test_find_neighbors_pbc_ext_subrange(hip, oracle, boundary):
     assert np.array_equal(nc0, ref_nc)
 
 
+@pytest.mark.parametrize("boundary", [(0, 0, 0), (1, 1, 1)])
+def test_find_neighbors_slot_major(hip, oracle, boundary):
+    """The slot-major layout holds the same rows: neighbor k of relative body r at ((r//64)*ngmax + k)*64 + r%64."""
+    T = np.float32
+    n = 30011
+    x, y, z, h = plummer_input(n, T, a=0.3, target=50)
+    gbox, obox = boxes(hip, T, -1.0, 1.0, boundary)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, np.uint64, 32, HILBERT)
+    first, last, ngmax = 77, 29000, 48
+    rows, counts = _gpu_neighbors(hip, p, gbox, first, last, ngmax)
+    d = {k: dev(hip, p[k]) for k in ("x", "y", "z", "h", "childOffsets", "parents", "internalToLeaf", "layout")}
+    m = last - first
+    groups = (m + 63) // 64
+    nb = hip.DeviceArray(groups * 64 * ngmax, np.uint32)
+    nb.fill(0xFF)
+    nc = hip.DeviceArray(m, np.uint32)
+    hip.find_neighbors(d["x"], d["y"], d["z"], d["h"], first, last, gbox, d["childOffsets"], d["parents"],
+                       d["internalToLeaf"], d["layout"], dev(hip, p["centers"].reshape(-1)),
+                       dev(hip, p["sizes"].reshape(-1)), p["prefixes"].size, ngmax, nb, nc, slot_major=True)
+    assert np.array_equal(nc.download(), counts)
+    sm = nb.download().reshape(groups, ngmax, 64).transpose(0, 2, 1).reshape(groups * 64, ngmax)[:m]
+    assert neighbor_rows_equal(sm, counts, rows, counts, ngmax) and counts.max() > ngmax
+
+
 def test_find_neighbors_coincident_and_tiny(hip, oracle):
     """Coincident particles are neighbors (only j == i is excluded, findneighbors.hpp:141); n < one wave."""
     T = np.float32
