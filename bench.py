@@ -290,7 +290,7 @@ def ijloop_bench(args):
 
     def step():
         hip.ijloop_example(v.x, v.y, v.z, v.h, dm, 0, n, box, v.child_offsets, v.parents, v.internal_to_leaf,
-                           v.layout, v.centers, v.sizes, v.num_nodes, ngmax, *outs)
+                           v.layout, v.centers, v.sizes, v.num_nodes, ngmax, *outs, total_bodies=n)
 
     for _ in range(W):
         step()

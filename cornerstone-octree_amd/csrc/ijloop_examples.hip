@@ -56,12 +56,15 @@ static hipcstone_status ijloopExample(hipcstone_stream s, const int32_t* childOf
                                       const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,
                                       const T* sizes, int32_t numNodes, const ijloop::Box<T>* box, uint32_t first,
                                       uint32_t last, const T* x, const T* y, const T* z, const T* h, const T* m,
-                                      uint32_t ngmax, float ext, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx)
+                                      uint32_t ngmax, float ext, uint32_t totalBodies, uint32_t numGroups,
+                                      const uint32_t* groupStart, const uint32_t* groupEnd, T* rho, uint32_t* count,
+                                      T* dmin, T* dmax, T* fx)
 {
     if (last <= first) return HIPCSTONE_OK;
     if (!childOffsets || !parents || !internalToLeaf || !layout || !centers || !sizes || !box || !x || !y || !z ||
-        !h || !m || !rho || !count || !dmin || !dmax || !fx)
+        !h || !m || !rho || !count || !dmin || !dmax || !fx || (numGroups && (!groupStart || !groupEnd)))
         return invalid("ijloop example: null pointer");
+    if (totalBodies < last) return invalid("ijloop example: total_bodies below last");
     try
     {
         ijloop::OctreeNsView<T, uint64_t> tree{};
@@ -74,10 +77,15 @@ static hipcstone_status ijloopExample(hipcstone_stream s, const int32_t* childOf
         tree.sizes           = reinterpret_cast<const ijloop::Vec3<T>*>(sizes);
         tree.searchExtFactor = ext;
         ijloop::GroupView groups{first, last, 0, nullptr, nullptr};
-        auto nb = ijloop::GpuAlwaysTraverseNeighborhoodBuilder{ngmax}.build(stream(s), tree, *box, last, groups, x,
-                                                                            y, z, h);
-        nb.ijLoop(std::make_tuple(m), std::make_tuple(rho, count, dmin, dmax, fx), ExampleInteraction<T>{},
-                  ExamplePostamble<T>{});
+        auto nb = ijloop::GpuAlwaysTraverseNeighborhoodBuilder{ngmax}.build(stream(s), tree, *box, totalBodies,
+                                                                            groups, x, y, z, h);
+        if (numGroups == 0)
+            nb.ijLoop(std::make_tuple(m), std::make_tuple(rho, count, dmin, dmax, fx), ExampleInteraction<T>{},
+                      ExamplePostamble<T>{});
+        else
+            nb.subgroup(ijloop::GroupView{first, last, numGroups, groupStart, groupEnd})
+                .ijLoop(std::make_tuple(m), std::make_tuple(rho, count, dmin, dmax, fx), ExampleInteraction<T>{},
+                        ExamplePostamble<T>{});
     }
     catch (std::exception const& e)
     {
@@ -99,10 +107,12 @@ extern "C"
         hipcstone_stream s, const int32_t* child_offsets, const int32_t* parents, const int32_t* internal_to_leaf,    \
         const uint32_t* layout, const T* centers, const T* sizes, int32_t num_nodes, const hipcstone_box_##RS* box,   \
         uint32_t first, uint32_t last, const T* x, const T* y, const T* z, const T* h, const T* m, uint32_t ngmax,    \
-        float ext, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx)                                                   \
+        float ext, uint32_t total_bodies, uint32_t num_groups, const uint32_t* group_start,                          \
+        const uint32_t* group_end, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx)                                   \
     {                                                                                                                  \
         return ijloopExample<T>(s, child_offsets, parents, internal_to_leaf, layout, centers, sizes, num_nodes, box,  \
-                                first, last, x, y, z, h, m, ngmax, ext, rho, count, dmin, dmax, fx);                   \
+                                first, last, x, y, z, h, m, ngmax, ext, total_bodies, num_groups, group_start,        \
+                                group_end, rho, count, dmin, dmax, fx);                                                \
     }
 HIPCSTONE_DEF_IJLOOP_EXAMPLE(f32, float)
 HIPCSTONE_DEF_IJLOOP_EXAMPLE(f64, double)

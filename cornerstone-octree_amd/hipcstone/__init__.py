@@ -399,15 +399,23 @@ def find_halos(prefixes, child_offsets, parents, centers, sizes, leaves, search_
 
 
 def ijloop_example(x, y, z, h, m, first, last, box, child_offsets, parents, internal_to_leaf, layout, centers, sizes,
-                   num_nodes, ngmax, rho, count, dmin, dmax, fx, ext=1.0, stream=None):
+                   num_nodes, ngmax, rho, count, dmin, dmax, fx, ext=1.0, total_bodies=None, groups=None,
+                   stream=None):
     """The fixed example interaction through the ijloop API (hipcstone_ijloop.hpp; reference
-    traversal/ijloop/gpu_alwaystraverse.cuh:174-204): outputs are body-indexed device arrays."""
+    traversal/ijloop/gpu_alwaystraverse.cuh:174-204): outputs are body-indexed device arrays. groups: optional
+    (num_groups, group_start, group_end) device arrays for the subgroup path."""
     T = np.dtype(box_dtype(box))
+    if total_bodies is None:
+        if not isinstance(x, DeviceArray):
+            raise ValueError("ijloop_example: pass total_bodies (the length of x..m) with raw device pointers")
+        total_bodies = x.count
+    total = total_bodies
+    ng, gs, ge = (0, None, None) if groups is None else groups
     f = _fn(f"hipcstone_ijloop_example_{_RS[T]}")
     _check(f(_stream(stream), _p(child_offsets), _p(parents), _p(internal_to_leaf), _p(layout), _p(centers), _p(sizes),
              C.c_int32(num_nodes), C.byref(box), C.c_uint32(first), C.c_uint32(last), _p(x), _p(y), _p(z), _p(h),
-             _p(m), C.c_uint32(ngmax), C.c_float(ext), _p(rho), _p(count), _p(dmin), _p(dmax), _p(fx)),
-           "ijloop_example")
+             _p(m), C.c_uint32(ngmax), C.c_float(ext), C.c_uint32(total), C.c_uint32(ng), _p(gs), _p(ge), _p(rho),
+             _p(count), _p(dmin), _p(dmax), _p(fx)), "ijloop_example")
 
 
 def _dtype_of(a, default=None):

@@ -391,13 +391,16 @@ HIPCSTONE_DECL_NEIGHBORS_SLOT_MAJOR(f64, double)
  * with w = (1 - d2 / ((4 h_i) h_i))^3: rho = sum m_j w / ((h_i h_i) h_i), count = 1 + neighbors used,
  * dmin / dmax = min / max of d2 over the neighbors, fx = sum of m_j (x_i - x_j) (minimum image for bodies whose 2h box
  * leaves a periodic domain). It equals the reference CPU ijloop (cpu_alwaystraverse.hpp:90-113) running the same
- * interaction. ---- */
+ * interaction. total_bodies is the length of x..m (>= last); with num_groups > 0 only the bodies of the device ranges
+ * [group_start[g], group_end[g]) (ascending, disjoint, inside [first, last)) are computed, through the
+ * neighborhood's subgroup(), like the reference's GroupView subgroups. ---- */
 #define HIPCSTONE_DECL_IJLOOP_EXAMPLE(RS, T)                                                                           \
     hipcstone_status hipcstone_ijloop_example_##RS(                                                                    \
         hipcstone_stream s, const int32_t* child_offsets, const int32_t* parents, const int32_t* internal_to_leaf,    \
         const uint32_t* layout, const T* centers, const T* sizes, int32_t num_nodes, const hipcstone_box_##RS* box,   \
         uint32_t first, uint32_t last, const T* x, const T* y, const T* z, const T* h, const T* m, uint32_t ngmax,    \
-        float ext, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx);
+        float ext, uint32_t total_bodies, uint32_t num_groups, const uint32_t* group_start,                          \
+        const uint32_t* group_end, T* rho, uint32_t* count, T* dmin, T* dmax, T* fx);
 HIPCSTONE_DECL_IJLOOP_EXAMPLE(f32, float)
 HIPCSTONE_DECL_IJLOOP_EXAMPLE(f64, double)
 

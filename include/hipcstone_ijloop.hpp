@@ -20,7 +20,9 @@
  * The neighbor search runs in the library (libhipcstone.so); the interaction kernel is instantiated here, in the
  * caller's translation unit, for the caller's interaction and postamble (device-callable functors or lambdas).
  * Bodies are processed in chunks so that the neighbor scratch stays below 2 GiB; the scratch holds each group of 64
- * bodies' lists slot-major (hipcstone_find_neighbors_slot_major_*), so the fold's per-slot index loads coalesce.
+ * bodies' lists slot-major (hipcstone_find_neighbors_slot_major_*), so the fold's per-slot index loads coalesce, and
+ * x, y, z, h of all totalBodies bodies are staged as one record per body at each ijLoop, so a neighbor's position
+ * and h come in one wide load (the values are copies, the results do not change).
  * h must be an array (the reference also accepts a constant). Errors are thrown as std::runtime_error carrying
  * hipcstone_last_error().
  */
@@ -208,12 +210,29 @@ __host__ __device__ constexpr auto unwrapModifiers(std::tuple<Ts...> const& v, s
     return std::make_tuple(unwrapOne(std::get<I>(v))...);
 }
 
-// (index, position, h, inputs...) of body i (common.hpp:39-55)
-template<class T, class... Ts, std::size_t... I>
-__device__ inline auto loadParticleData(const T* x, const T* y, const T* z, const T* h,
-                                        std::tuple<const Ts*...> const& input, LocalIndex i, std::index_sequence<I...>)
+// x, y, z, h of one body, staged as one record so that a neighbor's position and h come in one wide load
+template<class T>
+struct alignas(4 * sizeof(T)) Record
 {
-    return std::tuple<LocalIndex, Vec3<T>, T, Ts...>(i, Vec3<T>{{x[i], y[i], z[i]}}, h[i], std::get<I>(input)[i]...);
+    T x, y, z, h;
+};
+
+template<class T>
+__global__ __launch_bounds__(256) void stageRecordsKernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                          const T* __restrict__ z, const T* __restrict__ h,
+                                                          LocalIndex n, Record<T>* __restrict__ rec)
+{
+    LocalIndex i = LocalIndex(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) rec[i] = Record<T>{x[i], y[i], z[i], h[i]};
+}
+
+// (index, position, h, inputs...) of body i (common.hpp:39-55), from the staged records
+template<class T, class... Ts, std::size_t... I>
+__device__ inline auto loadParticleData(const Record<T>* rec, std::tuple<const Ts*...> const& input, LocalIndex i,
+                                        std::index_sequence<I...>)
+{
+    const Record<T> r = rec[i];
+    return std::tuple<LocalIndex, Vec3<T>, T, Ts...>(i, Vec3<T>{{r.x, r.y, r.z}}, r.h, std::get<I>(input)[i]...);
 }
 
 template<class... Ts, class... Vs, std::size_t... I>
@@ -252,8 +271,7 @@ __device__ inline void posDiffAndDistSq(bool usePbc, const Box<T>& box, const Ve
 }
 
 template<bool AnyPbc, class T, class... In, class... Out, class Interaction, class Postamble>
-__global__ __launch_bounds__(256) void ijLoopKernel(Box<T> box, const T* __restrict__ x, const T* __restrict__ y,
-                                                     const T* __restrict__ z, const T* __restrict__ h,
+__global__ __launch_bounds__(256) void ijLoopKernel(Box<T> box, const Record<T>* __restrict__ rec,
                                                      std::tuple<const In*...> input, std::tuple<Out*...> output,
                                                      Interaction interaction, Postamble postamble, unsigned ngmax,
                                                      const LocalIndex* __restrict__ neighbors,
@@ -261,7 +279,7 @@ __global__ __launch_bounds__(256) void ijLoopKernel(Box<T> box, const T* __restr
                                                      LocalIndex chunkLast, GroupView groups)
 {
     auto body = [&](LocalIndex i) {
-        const auto iData = loadParticleData(x, y, z, h, input, i, std::index_sequence_for<In...>{});
+        const auto iData = loadParticleData(rec, input, i, std::index_sequence_for<In...>{});
         const Vec3<T> pi = std::get<1>(iData);
         bool usePbc      = false;
         if constexpr (AnyPbc) usePbc = requiresPbc(box, pi, std::get<2>(iData));
@@ -274,7 +292,7 @@ __global__ __launch_bounds__(256) void ijLoopKernel(Box<T> box, const T* __restr
         for (unsigned nb = 0; nb < nbs; ++nb)
         {
             const LocalIndex j = slots[size_t(nb) * 64];
-            const auto jData   = loadParticleData(x, y, z, h, input, j, std::index_sequence_for<In...>{});
+            const auto jData   = loadParticleData(rec, input, j, std::index_sequence_for<In...>{});
             Vec3<T> r;
             T d2;
             posDiffAndDistSq(usePbc, box, pi, std::get<1>(jData), r, d2);
@@ -352,10 +370,12 @@ class GpuAlwaysTraverseNeighborhood
 public:
     static constexpr std::size_t kScratchElements = std::size_t(1) << 29; // neighbor slots per chunk (2 GiB)
 
+    // totalBodies: the length of x, y, z, h (neighbors lie anywhere in it)
     GpuAlwaysTraverseNeighborhood(hipStream_t stream, const OctreeNsView<T, KeyType>& tree, const Box<T>& box,
-                                  const GroupView& groups, const T* x, const T* y, const T* z, const T* h,
-                                  unsigned ngmax)
-        : stream_(stream), tree_(tree), box_(box), groups_(groups), x_(x), y_(y), z_(z), h_(h), ngmax_(ngmax)
+                                  LocalIndex totalBodies, const GroupView& groups, const T* x, const T* y, const T* z,
+                                  const T* h, unsigned ngmax)
+        : stream_(stream), tree_(tree), box_(box), groups_(groups), x_(x), y_(y), z_(z), h_(h), ngmax_(ngmax),
+          total_(std::max(totalBodies, groups.lastBody))
     {
         LocalIndex bodies = groups.lastBody > groups.firstBody ? groups.lastBody - groups.firstBody : 0;
         std::size_t fit   = std::max<std::size_t>(kScratchElements / std::max(ngmax, 1u) / 64 * 64, 64);
@@ -367,6 +387,9 @@ public:
         neighbors_.reset(static_cast<LocalIndex*>(p));
         detail::checkHip(hipMalloc(&p, std::size_t(chunk_) * sizeof(LocalIndex)), "ijloop: count scratch");
         counts_.reset(static_cast<LocalIndex*>(p));
+        detail::checkHip(hipMalloc(&p, std::max<std::size_t>(total_, 1) * sizeof(detail::Record<T>)),
+                         "ijloop: record scratch");
+        records_.reset(static_cast<detail::Record<T>*>(p));
     }
 
     template<class... In, class... Out, class Interaction, class Postamble>
@@ -380,7 +403,8 @@ public:
     Statistics stats() const
     {
         return {std::size_t(groups_.lastBody - groups_.firstBody),
-                std::size_t(chunk_) * (std::size_t(ngmax_) + 1) * sizeof(LocalIndex)};
+                std::size_t(chunk_) * (std::size_t(ngmax_) + 1) * sizeof(LocalIndex) +
+                    std::size_t(total_) * sizeof(detail::Record<T>)};
     }
 
     struct Subgroup
@@ -411,6 +435,9 @@ private:
         std::tuple<const std::remove_const_t<In>*...> cin = input;
         const bool anyPbc = box_.bnd[0] == HIPCSTONE_PERIODIC || box_.bnd[1] == HIPCSTONE_PERIODIC ||
                             box_.bnd[2] == HIPCSTONE_PERIODIC;
+        detail::stageRecordsKernel<T><<<unsigned((std::size_t(total_) + 255) / 256), 256, 0, stream_>>>(
+            x_, y_, z_, h_, total_, records_.get());
+        detail::checkHip(hipGetLastError(), "ijloop: record staging");
         for (LocalIndex c0 = first; c0 < last; c0 += std::min(chunk_, last - c0))
         {
             const LocalIndex c1 = c0 + std::min(chunk_, last - c0);
@@ -423,11 +450,11 @@ private:
                                                : unsigned((std::size_t(c1 - c0) + 255) / 256);
             if (anyPbc)
                 detail::ijLoopKernel<true, T><<<blocks, 256, 0, stream_>>>(
-                    box_, x_, y_, z_, h_, cin, output, interaction, postamble, ngmax_, neighbors_.get(),
+                    box_, records_.get(), cin, output, interaction, postamble, ngmax_, neighbors_.get(),
                     counts_.get(), c0, c1, groups);
             else
                 detail::ijLoopKernel<false, T><<<blocks, 256, 0, stream_>>>(
-                    box_, x_, y_, z_, h_, cin, output, interaction, postamble, ngmax_, neighbors_.get(),
+                    box_, records_.get(), cin, output, interaction, postamble, ngmax_, neighbors_.get(),
                     counts_.get(), c0, c1, groups);
             detail::checkHip(hipGetLastError(), "ijloop: interaction kernel");
         }
@@ -439,8 +466,10 @@ private:
     GroupView groups_;
     const T *x_, *y_, *z_, *h_;
     unsigned ngmax_;
+    LocalIndex total_;
     LocalIndex chunk_ = 1;
     std::unique_ptr<LocalIndex, detail::DeviceFree> neighbors_, counts_;
+    std::unique_ptr<detail::Record<T>, detail::DeviceFree> records_;
 };
 
 // gpu_alwaystraverse.cuh:180-206
@@ -450,11 +479,11 @@ struct GpuAlwaysTraverseNeighborhoodBuilder
 
     template<class T, class KeyType>
     GpuAlwaysTraverseNeighborhood<T, KeyType> build(hipStream_t stream, const OctreeNsView<T, KeyType>& tree,
-                                                     const Box<T>& box, LocalIndex /* totalBodies */,
+                                                     const Box<T>& box, LocalIndex totalBodies,
                                                      const GroupView& groups, const T* x, const T* y, const T* z,
                                                      const T* h) const
     {
-        return GpuAlwaysTraverseNeighborhood<T, KeyType>(stream, tree, box, groups, x, y, z, h, ngmax);
+        return GpuAlwaysTraverseNeighborhood<T, KeyType>(stream, tree, box, totalBodies, groups, x, y, z, h, ngmax);
     }
 };
 

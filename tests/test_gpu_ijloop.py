@@ -26,7 +26,7 @@ def _case(oracle, T, n, bnd, seed):
     return p, obox, m
 
 
-def _gpu(hip, p, T, bnd, m, first, last, ngmax, ext=1.0):
+def _gpu(hip, p, T, bnd, m, first, last, ngmax, ext=1.0, groups=None):
     gbox = hip.make_box((-1.0, -1.05, -1.0), (1.0, 1.0, 1.1), T, bnd)
     d = {k: dev(hip, p[k]) for k in ("x", "y", "z", "h", "childOffsets", "parents", "internalToLeaf", "layout")}
     dc, ds = dev(hip, p["centers"].reshape(-1)), dev(hip, p["sizes"].reshape(-1))
@@ -37,7 +37,7 @@ def _gpu(hip, p, T, bnd, m, first, last, ngmax, ext=1.0):
         o.fill(0)
     hip.ijloop_example(d["x"], d["y"], d["z"], d["h"], dev(hip, m), first, last, gbox, d["childOffsets"],
                        d["parents"], d["internalToLeaf"], d["layout"], dc, ds, p["prefixes"].size, ngmax, *outs,
-                       ext=ext)
+                       ext=ext, groups=groups)
     return [o.download() for o in outs]
 
 
@@ -67,3 +67,23 @@ def test_ijloop_example_gpu_chunked(hip, oracle):
     got = _gpu(hip, p, T, (0, 0, 0), m, 0, 30000, 65536)
     for name, g, w in zip(("rho", "count", "dmin", "dmax", "fx"), got, want):
         assert np.array_equal(g, w), name
+
+
+def test_ijloop_example_gpu_subgroups(hip, oracle):
+    """subgroup(): only the bodies of the given ranges are computed, with the same values as the full loop."""
+    T = np.float64
+    print("ijloop subgroups", flush=True)
+    p, obox, m = _case(oracle, T, 20000, (1, 1, 1), seed=14)
+    tree = {k: p[k] for k in ("childOffsets", "parents", "internalToLeaf")}
+    first, last = 200, 19000
+    want = oracle.ijloop_example(p["x"], p["y"], p["z"], p["h"], m, first, last, obox, tree, p["layout"],
+                                 p["centers"], p["sizes"], 32)
+    starts = np.array([210, 500, 4001, 12000], dtype=np.uint32)
+    ends = np.array([300, 3000, 9000, 18999], dtype=np.uint32)
+    got = _gpu(hip, p, T, (1, 1, 1), m, first, last, 32, groups=(4, dev(hip, starts), dev(hip, ends)))
+    inside = np.zeros(20000, bool)
+    for b, e in zip(starts, ends):
+        inside[b:e] = True
+    for name, g, w in zip(("rho", "count", "dmin", "dmax", "fx"), got, want):
+        assert np.array_equal(g[inside], w[inside]), name
+        assert not g[~inside].any(), name
