@@ -351,3 +351,49 @@ def test_bbox_and_halos(hip, oracle, kd, T):
     with pytest.raises(hip.HipcstoneError):
         hip.find_halos(d["prefixes"], d["childOffsets"], d["parents"], dc, dsz, d["leaves"], dsc, dss, gbox, first,
                        last, num_nodes, flags, workspace=ws, workspace_bytes=wsb - 1)
+
+
+# ---------------------------------------------------------------- findHalos edge cases
+
+
+def _halo_case(hip, oracle, tree, T, boxlim, first, last, sizes_scale, seed):
+    """findHalos on a given leaf array with random search boxes, GPU against the oracle."""
+    kd = tree.dtype
+    o = oracle.build_octree(tree)
+    obox = orc_box(boxlim[0], boxlim[1], T)
+    gbox = hip.make_box(boxlim[0], boxlim[1], T)
+    c, s = oracle.node_fp_centers(o["prefixes"], obox, T)
+    L = tree.size - 1
+    N = o["prefixes"].size
+    rng = np.random.default_rng(seed)
+    NI = N - L
+    sc = c[o["leafToInternal"][NI:]].copy()
+    ss = (s[o["leafToInternal"][NI:]] * (1.0 + sizes_scale * rng.random((L, 3)))).astype(T)
+    want = oracle.find_halos(o["prefixes"], o["childOffsets"], o["parents"], c, s, tree, sc, ss, obox, first, last)
+    flags = hip.DeviceArray(N, np.uint8)
+    flags.fill(0)
+    hip.find_halos(dev(hip, o["prefixes"]), dev(hip, o["childOffsets"]), dev(hip, o["parents"]),
+                   dev(hip, c.reshape(-1)), dev(hip, s.reshape(-1)), dev(hip, tree), dev(hip, sc.reshape(-1)),
+                   dev(hip, ss.reshape(-1)), gbox, first, last, N, flags)
+    got = flags.download()
+    assert np.array_equal(got, want)
+    return got
+
+
+@pytest.mark.parametrize("kd", KDS)
+def test_find_halos_edge_cases(hip, oracle, kd):
+    T = np.float64
+    # root-only tree: the single leaf is local, nothing is outside the local range
+    root = np.array([0, int(1 << (63 if kd == np.uint64 else 30))], dtype=kd)
+    assert not _halo_case(hip, oracle, root, T, (0.0, 1.0), 0, 1, 0.5, 1).any()
+    # one level: every leaf local (one rank) flags nothing; an empty local range flags nothing
+    t1 = uniform_n_level_tree(kd, 1)
+    assert not _halo_case(hip, oracle, t1, T, (0.0, 1.0), 0, 8, 2.0, 2).any()
+    assert not _halo_case(hip, oracle, t1, T, (0.0, 1.0), 3, 3, 2.0, 3).any()
+    # one local leaf of a one-level tree with boxes reaching its neighbours
+    assert _halo_case(hip, oracle, t1, T, (0.0, 1.0), 3, 4, 2.0, 4).sum() > 1
+    # an irregular tree, local range in the middle, boxes of up to 3x the leaf size
+    tree = OctreeMaker(kd).divide().divide(0).divide(3).divide(0, 4).divide(7).divide(7, 2).make_tree()
+    L = tree.size - 1
+    for first, last in ((0, L), (1, L - 1), (L // 3, L // 2), (L - 1, L)):
+        _halo_case(hip, oracle, tree, T, (-1.0, 2.0), first, last, 3.0, first + 7)
