@@ -303,6 +303,9 @@ __host__ __device__ inline K sfc3D(T x, T y, T z, const Box& box)
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// this lane's bit of a wave-uniform mask; the mask is used as a lane predicate as is (no per-lane shift and compare)
+__device__ inline bool laneIn(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
+
 // number of set bits of mask below this lane
 __device__ inline unsigned laneRank(uint64_t mask)
 {

@@ -56,6 +56,15 @@ constexpr SlotTables makeSlotTables()
 
 __constant__ SlotTables kSlotTables = makeSlotTables();
 
+// v + (this lane's bit of the wave-uniform mask) as one add with the mask as carry-in
+__device__ inline uint32_t addLaneBit(uint32_t v, uint64_t mask)
+{
+    uint32_t r;
+    uint64_t carryOut;
+    asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(carryOut) : "v"(v), "s"(mask));
+    return r;
+}
+
 // dword 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family targets: 32-bit data format, range checked
 // against the byte count in dword 2
 constexpr int kRawBufferFlags = 0x00020000;
@@ -173,15 +182,17 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     //      hit among its target's hits in ascending j and all of a target's hits of the chunk land contiguously in
     //      its row (coalesced runs). The slot's last lane advances the target's count. ----
     int listCount = 0;
-    auto drain = [&]() {
+    // drain and its pair-loop body are forced inline: as out-of-line calls their captured state (the row
+    // buffer resource among it) would live in scratch memory
+    auto drain = [&]() __attribute__((always_inline)) {
         for (int e = 0; e < listCount; ++e)
         {
             int32_t leafIdx = uniform(sh.listLeaf[w][e]);
             uint64_t m      = uniform64(sh.listMask[w][e]);
             uint32_t jb     = uniform(layout[leafIdx]);
             uint32_t je     = uniform(layout[leafIdx + 1]);
-            int pc          = __builtin_popcountll(m);
-            const bool mine = (m >> lane) & 1u;
+            const uint32_t pc     = uint32_t(__builtin_popcountll(m));
+            const bool mine       = laneIn(m);
             const unsigned myRank = laneRank(m);
             TargetRec<T>* recs    = &sh.ctgt[w * kWave];
             if (mine)
@@ -190,6 +201,11 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 r.wpos         = myWpos;
                 recs[myRank]   = r;
             }
+            // no active target can reach the end of its row in this leaf (at most je - jb hits each): the
+            // per-hit row-end compare is skipped. wpos keeps counting past the row end, so compare in 64 bits.
+            const bool fits =
+                uint64_t(myWpos) + uint64_t(je - jb) * kSlotStep <= uint64_t(myRec.limit);
+            const bool noOverflow = (ballot(!fits) & m) == 0;
             waveSync();
             for (uint32_t cs = jb; cs < je; cs += kWave)
             {
@@ -204,9 +220,13 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 const uint64_t notBelow = ~((uint64_t(1) << (slot * nj)) - 1);
                 const uint32_t nbLo     = uint32_t(notBelow);
                 const uint32_t nbHi     = uint32_t(notBelow >> 32);
-                // the slot's last lane advances the count of the slot's target: the write condition last & (k < pc)
-                // is the single compare k < pcIfLast
-                const int pcIfLast = (jv && jj == nj - 1) ? pc : 0;
+                // the slots' last lanes: each advances the count of its slot's target
+                const uint64_t lastLanes = ballot(jv && jj == nj - 1);
+                // lanes of the first b slots, [0, b * nj), as a wave-uniform mask (scalar arithmetic)
+                auto slotsBelow = [&](uint32_t b) -> uint64_t {
+                    const uint32_t bits = b * nj;
+                    return bits >= uint32_t(kWave) ? ~uint64_t(0) : (uint64_t(1) << bits) - 1;
+                };
                 // lanes past the last slot test a NaN particle, which fails every distance compare
                 T xj = std::numeric_limits<T>::quiet_NaN(), yj = xj, zj = xj;
                 if (jv)
@@ -249,44 +269,80 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                                                          __builtin_amdgcn_mbcnt_lo(uint32_t(H) & nbLo, wpos));
                     }
                 };
-                auto iterate = [&](auto checkSelf) {
-                    // records of the slot's two targets of an iteration, walked by pointer: k < pc is
-                    // &recs[k] < recs + pc, and the count write condition is &recs[k] < writeEnd
-                    TargetRec<T>* rA               = recs + slot; // index < 128: pc <= 64 and slot < S <= 64
-                    const TargetRec<T>* const end  = recs + pc;
-                    const TargetRec<T>* writeEnd   = recs + pcIfLast;
-                    for (int k0 = 0; k0 < pc; k0 += 2 * int(S), rA += 2 * S)
-                    {
+                // the slot position after a lane's own pair: pos, plus one slot if the lane's bit of H is set
+                auto advance = [&](uint32_t pos, uint64_t H) -> uint32_t {
+                    if constexpr (SlotMajor) { return pos + (laneIn(H) ? kSlotStep : 0u); }
+                    else { return addLaneBit(pos, H); }
+                };
+                auto iterate = [&](auto checkSelf, auto checkLimit) {
+                    // hits of particle j against a target record, among the lanes of `in`
+                    auto hits = [&](const TargetRec<T>& rec, uint64_t in) -> uint64_t {
+                        uint64_t H = ballot(close(rec)) & in;
+                        if constexpr (decltype(checkSelf)::value) { H &= ballot(jRel != (rec.meta >> 8)); }
+                        return H;
+                    };
+                    // store byte offset of a lane's pair: its row slot for a hit (inside the row), else past the
+                    // end of the buffer, where the store is dropped
+                    auto storeOffset = [&](const TargetRec<T>& rec, uint64_t H, uint32_t pos) -> uint32_t {
+                        if constexpr (decltype(checkLimit)::value)
+                        {
+                            return (laneIn(H) & (pos < rec.limit)) ? pos * 4u : rowsBytes;
+                        }
+                        else { return laneIn(H) ? pos * 4u : rowsBytes; }
+                    };
+                    // slot s of an iteration holds targets k0 + s (A) and k0 + S + s (B). In the iterations with
+                    // k0 + 2S <= pc every slot holds a target (lanes past the last slot test a NaN particle, which
+                    // fails every compare), so no lane mask applies; the last, partial iteration masks the slots
+                    // past pc with wave-uniform masks, and has no B half if pc - k0 <= S. Both records are read
+                    // before either count is written back.
+                    auto body2 = [&](TargetRec<T>* rA, uint64_t inA, uint64_t inB, uint64_t wA,
+                                     uint64_t wB) __attribute__((always_inline)) {
                         TargetRec<T>* rB        = rA + S;
-                        const bool inA          = rA < end;
-                        const bool inB          = rB < end;
                         const TargetRec<T> recA = *rA;
                         const TargetRec<T> recB = *rB;
-                        bool cA = close(recA), cB = close(recB);
-                        uint64_t HA = ballot(cA) & ballot(inA);
-                        uint64_t HB = ballot(cB) & ballot(inB);
-                        if constexpr (decltype(checkSelf)::value)
-                        {
-                            const bool oA = jRel != (recA.meta >> 8), oB = jRel != (recB.meta >> 8);
-                            HA &= ballot(oA);
-                            HB &= ballot(oB);
-                            cA = cA & oA;
-                            cB = cB & oB;
-                        }
-                        const bool hitA     = inA & cA;
-                        const bool hitB     = inB & cB;
-                        const uint32_t posA = position(HA, recA.wpos);
-                        const uint32_t posB = position(HB, recB.wpos);
-                        const uint32_t offA = (hitA & (posA < recA.limit)) ? posA * 4u : rowsBytes;
-                        const uint32_t offB = (hitB & (posB < recB.limit)) ? posB * 4u : rowsBytes;
-                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offA), 0, 0);
-                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(offB), 0, 0);
-                        if (rA < writeEnd) rA->wpos = posA + (hitA ? kSlotStep : 0u);
-                        if (rB < writeEnd) rB->wpos = posB + (hitB ? kSlotStep : 0u);
+                        const uint64_t HA       = hits(recA, inA);
+                        const uint64_t HB       = hits(recB, inB);
+                        const uint32_t posA     = position(HA, recA.wpos);
+                        const uint32_t posB     = position(HB, recB.wpos);
+                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0);
+                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(storeOffset(recB, HB, posB)), 0, 0);
+                        if (laneIn(wA)) rA->wpos = advance(posA, HA);
+                        if (laneIn(wB)) rB->wpos = advance(posB, HB);
+                    };
+                    auto body1 = [&](TargetRec<T>* rA, uint64_t inA, uint64_t wA) __attribute__((always_inline)) {
+                        const TargetRec<T> recA = *rA;
+                        const uint64_t HA       = hits(recA, inA);
+                        const uint32_t posA     = position(HA, recA.wpos);
+                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0);
+                        if (laneIn(wA)) rA->wpos = advance(posA, HA);
+                    };
+                    TargetRec<T>* rA = recs + slot; // index < 128: pc <= 64 and slot <= S <= 64
+                    uint32_t left    = pc;
+                    for (; left >= 2 * S; left -= 2 * S, rA += 2 * S)
+                    {
+                        body2(rA, ~uint64_t(0), ~uint64_t(0), lastLanes, lastLanes);
+                    }
+                    if (left > S)
+                    {
+                        const uint64_t inB = slotsBelow(left - S);
+                        body2(rA, ~uint64_t(0), inB, lastLanes, lastLanes & inB);
+                    }
+                    else if (left > 0)
+                    {
+                        const uint64_t inA = slotsBelow(left);
+                        body1(rA, inA, lastLanes & inA);
                     }
                 };
-                if (selfChunk) iterate(std::true_type{});
-                else iterate(std::false_type{});
+                if (selfChunk)
+                {
+                    if (noOverflow) iterate(std::true_type{}, std::false_type{});
+                    else iterate(std::true_type{}, std::true_type{});
+                }
+                else
+                {
+                    if (noOverflow) iterate(std::false_type{}, std::false_type{});
+                    else iterate(std::false_type{}, std::true_type{});
+                }
                 waveSync();
             }
             if (mine) myWpos = recs[myRank].wpos;
