@@ -10,8 +10,8 @@
 //  * Walk: depth-first with an explicit per-wave stack in LDS. Descending into a node evaluates its 8 children at
 //    once (their centers/sizes are contiguous, so a few wide scalar loads fetch them) and stores, per child, the
 //    ballot of lanes whose own predicate chain passed: a lane is tested at a child only if it passed the parent.
-//    The frame also keeps each child's first child (or, for a leaf, its leaf index), fetched in the same batch of
-//    loads, so each step of the walk waits on one memory round trip instead of two.
+//    A frame holds the children's lane masks and the first child's index; stepping to a child loads its child
+//    offset (and, for a leaf, its leaf index) with scalar loads.
 //    The visited leaf set of every lane is therefore exactly the CPU's, not the superset a group-box walk gives.
 //    Children are taken in order 0..7, so leaves come out in the CPU's visiting order (ascending SFC order).
 //  * Leaves and their lane masks go to a per-wave list, drained in visiting order, so every target's neighbors come
@@ -175,12 +175,14 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         __builtin_amdgcn_make_buffer_rsrc(ngmax ? static_cast<void*>(neighbors + rowBase * ngmax) : nullptr, 0,
                                           int(rowsBytes), kRawBufferFlags);
 
-    // ---- leaf phase. A leaf chunk of nj <= 64 particles is tested against its active targets in pair slots:
+    // ---- leaf phase. A leaf chunk of nj <= 32 particles is tested against its active targets in pair slots:
     //      lane = (slot, jj) with slot = lane / nj, jj = lane % nj, S = 64 / nj slots; slot s of an iteration tests
     //      active targets k = k0 + s and k0 + S + s (two independent targets per lane, for instruction-level
     //      parallelism) against particle jj. Within a slot, lanes are ascending jj, so the ballot of hits ranks each
     //      hit among its target's hits in ascending j and all of a target's hits of the chunk land contiguously in
-    //      its row (coalesced runs). The slot's last lane advances the target's count. ----
+    //      its row (coalesced runs). The slot's last lane advances the target's count. Chunks are taken in order,
+    //      at most 32 particles each: a leaf of 33-64 particles as one chunk would have one slot per iteration,
+    //      as 32 + the rest it has at least two. ----
     int listCount = 0;
     // drain and its pair-loop body are forced inline: as out-of-line calls their captured state (the row
     // buffer resource among it) would live in scratch memory
@@ -207,9 +209,9 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 uint64_t(myWpos) + uint64_t(je - jb) * kSlotStep <= uint64_t(myRec.limit);
             const bool noOverflow = (ballot(!fits) & m) == 0;
             waveSync();
-            for (uint32_t cs = jb; cs < je; cs += kWave)
+            for (uint32_t cs = jb, nj = 0; cs < je; cs += nj)
             {
-                uint32_t nj    = je - cs < uint32_t(kWave) ? je - cs : uint32_t(kWave);
+                nj             = je - cs < 32u ? je - cs : 32u;
                 uint32_t magic = uniform(kSlotTables.magic[nj]);
                 uint32_t S     = uniform(kSlotTables.slots[nj]);
                 uint32_t slot  = (uint32_t(lane) * magic) >> 16;
