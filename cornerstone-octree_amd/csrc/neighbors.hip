@@ -187,6 +187,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     // drain and its pair-loop body are forced inline: as out-of-line calls their captured state (the row
     // buffer resource among it) would live in scratch memory
     auto drain = [&]() __attribute__((always_inline)) {
+        waveSync();
         for (int e = 0; e < listCount; ++e)
         {
             int32_t leafIdx = uniform(sh.listLeaf[w][e]);
@@ -353,29 +354,38 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         listCount = 0;
     };
 
+    // leaves go to the wave's list (drained before it can overflow: the walk drains it whenever fewer than 9 free
+    // entries are left, and one descent appends at most 8 leaves); the drain reads the list after a wave sync
     auto appendLeaf = [&](int32_t leafIdx, uint64_t m) {
         if (lane == 0)
         {
             sh.listLeaf[w][listCount] = leafIdx;
             sh.listMask[w][listCount] = m;
         }
-        waveSync();
         listCount++;
-        if (listCount == kLeafList) drain();
     };
 
-    // evaluate the 8 children of a node for the lanes in `pm` and push them as a stack frame; the 48 node values
-    // and the children's first-child / leaf indices are loaded up front (contiguous, wave-uniform) and the tests
-    // are branch-free
-    auto pushFrame = [&](int depth, int32_t co, uint64_t pm) {
-        const T* c          = centers + size_t(co) * 3;
-        const T* s          = sizes + size_t(co) * 3;
+    // evaluate the 8 children of node range [co, co + 8) for the lanes in `pm`. If every child that some lane
+    // passes is a leaf, those leaves are appended in child order and no frame is pushed (true of most nodes at
+    // the bottom of the tree); otherwise the children's lane masks are pushed as the frame at `depth`. The 48 node
+    // values and the 16 child offsets / leaf indices are loaded up front (contiguous, wave-uniform) and the tests
+    // are branch-free. Returns whether a frame was pushed.
+    auto descend = [&](int depth, int32_t co, uint64_t pm) -> bool {
+        const T* c = centers + size_t(co) * 3;
+        const T* s = sizes + size_t(co) * 3;
         T cv[24], sv[24];
+        int32_t cc[8], lf[8];
 #pragma unroll
         for (int k = 0; k < 24; ++k)
         {
             cv[k] = c[k];
             sv[k] = s[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            cc[k] = childOffsets[co + k];
+            lf[k] = internalToLeaf[co + k];
         }
         uint64_t mk[8];
         // the ballot of the bare compare is the compare's own lane mask; the parent's lanes are applied as a
@@ -385,6 +395,21 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
         {
             T d2  = nodeDist2(cv[3 * k], cv[3 * k + 1], cv[3 * k + 2], sv[3 * k], sv[3 * k + 1], sv[3 * k + 2]);
             mk[k] = ballot(d2 < cellR2) & pm;
+        }
+        bool allLeaves = true;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            allLeaves = allLeaves && (mk[k] == 0 || cc[k] == 0);
+        }
+        if (allLeaves)
+        {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+            {
+                if (mk[k] != 0) appendLeaf(lf[k], mk[k]);
+            }
+            return false;
         }
         if (lane == 0)
         {
@@ -397,6 +422,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
             sh.next[w][depth] = 0;
         }
         waveSync();
+        return true;
     };
 
     // root (traversal.hpp:35-45)
@@ -406,13 +432,13 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     {
         int32_t rootChild = uniform(childOffsets[0]);
         if (rootChild == 0) { appendLeaf(uniform(internalToLeaf[0]), rootMask); }
-        else
+        else if (rootChild > 0 && rootChild + 8 <= numNodes) // malformed tree guard
         {
-            int depth = 0;
-            pushFrame(0, rootChild, rootMask);
+            int depth         = descend(0, rootChild, rootMask) ? 0 : -1;
             int64_t stepsLeft = 2 * int64_t(numNodes) + 8;
             while (depth >= 0 && stepsLeft-- > 0)
             {
+                if (listCount > kLeafList - 9) drain();
                 int k      = uniform(sh.next[w][depth]);
                 uint64_t m = 0;
                 while (k < 8)
@@ -435,8 +461,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 else
                 {
                     if (depth + 1 >= kMaxDepth || cco <= node || cco + 8 > numNodes) break;
-                    ++depth;
-                    pushFrame(depth, cco, m);
+                    if (descend(depth + 1, cco, m)) ++depth;
                 }
             }
         }
