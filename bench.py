@@ -83,12 +83,27 @@ def cpu_baseline(x, y, z, h, stride, seed):
     }
 
 
-def load_traffic():
-    p = os.path.join(ROOT, "profiles", "neighbors_c2_traffic.json")
+def load_profile(name):
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f)
     return None
+
+
+def load_traffic():
+    return load_profile("neighbors_c2_traffic.json")
+
+
+def valu_busy(avg_launch_ms):
+    """Fraction of the device's VALU issue capacity the neighbor kernel uses: its wave64 VALU instruction count per
+    launch (SQ_INSTS_VALU, committed counter summary) x 4 cycles each / (live launch time x clock x SIMDs)."""
+    mix = load_profile("neighbors_c2_instruction_mix.json")
+    if not mix or avg_launch_ms <= 0:
+        return None
+    cycles = mix["wave_instructions_per_launch"]["valu"] * mix["valu_cycles_per_wave_instruction"]
+    capacity = avg_launch_ms / 1e3 * mix["engine_clock_ghz"] * 1e9 * mix["simds"]
+    return round(cycles / capacity, 3)
 
 
 def halo_bench(args):
@@ -494,6 +509,8 @@ def main():
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
             "avg_launch_ms": round(nb_avg, 3),
             "algorithmic_bytes_per_launch": nb_bytes,
+            # the kernel is issue-bound on the VALU, not on HBM: share of the VALU issue capacity it keeps busy
+            "valu_busy": valu_busy(nb_avg),
         },
         "pipeline_roofline": {
             "algorithmic_bytes_per_step": step_bytes,

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 nj             = je - cs < 32u ? je - cs : 32u;
                 uint32_t magic = uniform(kSlotTables.magic[nj]);
                 uint32_t S     = uniform(kSlotTables.slots[nj]);
-                uint32_t slot  = (uint32_t(lane) * magic) >> 16;
+                uint32_t slot  = __umul24(uint32_t(lane), magic) >> 16; // lane < 64, magic <= 2^16
                 uint32_t jj    = uint32_t(lane) - slot * nj;
                 bool jv        = slot < S;
                 uint32_t j     = cs + jj;
