@@ -293,6 +293,49 @@ def test_find_neighbors_slot_major(hip, oracle, boundary):
     assert neighbor_rows_equal(sm, counts, rows, counts, ngmax) and counts.max() > ngmax
 
 
+@pytest.mark.parametrize("T", TS)
+@pytest.mark.parametrize("kd", KDS)
+def test_find_neighbors_dense_leaves(hip, oracle, T, kd):
+    """Leaves far above the bucket size: a 400-particle cluster narrower than a max-depth cell and 100 coincident
+    particles end up in max-depth leaves (csarray.cpp:159-190), so a leaf is drained in many particle chunks, its own
+    targets sit in several of them, and rows fill before the leaf ends (ngmax 150 and 32 against ~400 neighbors)."""
+    rng = np.random.default_rng(5)
+    x0, y0, z0, h0 = plummer_input(6000, T, a=0.2, target=60)
+    cx = (0.3 + rng.random(400) * 2e-7).astype(T)
+    cy = (-0.2 + rng.random(400) * 2e-7).astype(T)
+    cz = (0.1 + rng.random(400) * 2e-7).astype(T)
+    x = np.concatenate([x0, cx, np.full(100, -0.4, T)])
+    y = np.concatenate([y0, cy, np.full(100, 0.4, T)])
+    z = np.concatenate([z0, cz, np.full(100, 0.0, T)])
+    h = np.concatenate([h0, np.full(400, 1e-3, T), np.full(100, 2e-3, T)])
+    order = rng.permutation(x.size)
+    x, y, z, h = x[order], y[order], z[order], h[order]
+    n = x.size
+    gbox, obox = boxes(hip, T, -1.0, 1.0)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, kd, 64, HILBERT)
+    assert np.diff(p["layout"]).max() >= 100
+    tree = {k: p[k] for k in ("childOffsets", "parents", "internalToLeaf")}
+    for ngmax in (150, 32):
+        ref_nb, ref_nc = oracle.find_neighbors(p["x"], p["y"], p["z"], p["h"], 0, n, obox, tree, p["layout"],
+                                               p["centers"], p["sizes"], ngmax)
+        nb, nc = _gpu_neighbors(hip, p, gbox, 0, n, ngmax)
+        assert neighbor_rows_equal(nb, nc, ref_nb, ref_nc, ngmax)
+        assert ref_nc.max() > 150
+
+
+@pytest.mark.parametrize("n", [1, 50, 64])
+def test_find_neighbors_root_leaf(hip, oracle, n):
+    """n <= bucket: the tree is the root alone, a leaf (traversal.hpp:35-45 visits it directly)."""
+    T = np.float32
+    x, y, z, h = plummer_input(n, T, a=0.3, target=20)
+    h = np.maximum(h, T(0.05))
+    gbox, obox = boxes(hip, T, -1.0, 1.0)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, np.uint64, 64, HILBERT, ngmax=16)
+    assert p["prefixes"].size == 1
+    nb, nc = _gpu_neighbors(hip, p, gbox, 0, n, 16)
+    assert neighbor_rows_equal(nb, nc, p["nb"], p["nc"], 16)
+
+
 def test_find_neighbors_coincident_and_tiny(hip, oracle):
     """Coincident particles are neighbors (only j == i is excluded, findneighbors.hpp:141); n < one wave."""
     T = np.float32
