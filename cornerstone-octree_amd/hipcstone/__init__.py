@@ -362,6 +362,42 @@ def compute_geo_centers(prefixes, box):
     return c.download().reshape(N, 3), s.download().reshape(N, 3)
 
 
+def leaf_mass_centers(x, y, z, m, leaf_to_internal, layout, centers, stream=None):
+    """Leaf mass centers (source_center_gpu.h:66-74, the CPU path's summation order) into the device array
+    `centers` (4 values per node) at leaf_to_internal[i] for every leaf i; all arguments are DeviceArrays."""
+    T = np.dtype(x.dtype)
+    f = _fn(f"hipcstone_leaf_mass_centers_{_RS[T]}")
+    _check(f(stream, _p(x), _p(y), _p(z), _p(m), _p(leaf_to_internal), C.c_int32(leaf_to_internal.count),
+             _p(layout), _p(centers)), "leaf_mass_centers")
+
+
+def upsweep_centers(level_range, child_offsets, centers, stream=None):
+    """Mass centers of the internal nodes from their children, level by level (source_center_gpu.h:86-90);
+    level_range is a host array of num_levels + 2 entries, child_offsets and centers DeviceArrays."""
+    lr = np.ascontiguousarray(level_range, dtype=np.int32)
+    T = np.dtype(centers.dtype)
+    f = _fn(f"hipcstone_upsweep_centers_{_RS[T]}")
+    _check(f(stream, C.c_int32(lr.size - 2), lr.ctypes.data_as(C.c_void_p), _p(child_offsets), _p(centers)),
+           "upsweep_centers")
+
+
+def set_mac(prefixes, centers, inv_theta, box, stream=None):
+    """centers[i][3] = squared vector-MAC radius where the mass is nonzero (source_center_gpu.h:110-116);
+    prefixes and centers are DeviceArrays."""
+    T = box_dtype(box)
+    f = _fn(f"hipcstone_set_mac_{_KS[np.dtype(prefixes.dtype)]}_{_RS[np.dtype(T)]}")
+    _check(f(stream, _p(prefixes), C.c_int32(prefixes.count), _p(centers), C.c_float(inv_theta), C.byref(box)),
+           "set_mac")
+
+
+def geo_mac_spheres(prefixes, centers, inv_theta, box, stream=None):
+    """centers[i] = {geometric center, squared min-MAC radius} (source_center_gpu.h:102-108)."""
+    T = box_dtype(box)
+    f = _fn(f"hipcstone_geo_mac_spheres_{_KS[np.dtype(prefixes.dtype)]}_{_RS[np.dtype(T)]}")
+    _check(f(stream, _p(prefixes), C.c_int32(prefixes.count), _p(centers), C.c_float(inv_theta), C.byref(box)),
+           "geo_mac_spheres")
+
+
 def leaf_layout(counts, stream=None):
     """layout = [0, inclusive_scan(counts)] of a device counts array; returns a DeviceArray of L+1."""
     L = counts.count

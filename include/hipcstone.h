@@ -223,7 +223,7 @@ hipcstone_status hipcstone_build_octree_u64(hipcstone_stream s, const uint64_t* 
                                             int32_t* level_range_host);
 
 /* ---- a11: geometric node centers and half sizes (Vec3 arrays, 3 T per node). Replaces
- * computeGeoCentersGpu(exec, prefixes, numNodes, centers, sizes, box), focus/source_center_gpu.h:93-99.
+ * computeGeoCentersGpu(exec, prefixes, numNodes, centers, sizes, box), focus/source_center_gpu.h:94-99.
  * As in the reference (source_center.hpp:145-158 via sfcKey -> SfcKind), node keys are decoded as Hilbert
  * keys. ---- */
 #define HIPCSTONE_DECL_CENTERS(KS, K, RS, T)                                                                           \
@@ -234,6 +234,42 @@ HIPCSTONE_DECL_CENTERS(u32, uint32_t, f32, float)
 HIPCSTONE_DECL_CENTERS(u32, uint32_t, f64, double)
 HIPCSTONE_DECL_CENTERS(u64, uint64_t, f32, float)
 HIPCSTONE_DECL_CENTERS(u64, uint64_t, f64, double)
+
+/* ---- Focus-tree source centers and MAC radii (the expansion centers of Domain::syncGrav and
+ * Domain::updateExpansionCenters, domain/domain.hpp:220-287, :366-372). centers holds 4 T per node: mass center
+ * x, y, z and total |m| (setMac replaces the last by the squared MAC radius).
+ * hipcstone_leaf_mass_centers replaces computeLeafSourceCenterGpu(exec, x, y, z, m, leafToInternal, numLeaves, layout,
+ * centers), focus/source_center_gpu.h:66-74, with the CPU path's summation order (source_center.hpp:72-78, 110-125:
+ * a leaf's particles [layout[i], layout[i+1]) in index order, weights |m|): centers[leaf_to_internal[i]] for every
+ * leaf i; leaf_to_internal is the leaf part of the map (leafToInternal + numInternalNodes).
+ * hipcstone_upsweep_centers replaces upsweepCentersGpu(exec, numLevels, levelRange, childOffsets, centers),
+ * source_center_gpu.h:86-90 (CombineSourceCenter, source_center.hpp:82-98): levels num_levels ... 0 of the host array
+ * level_range_host (num_levels + 2 entries, as hipcstone_build_octree_* returns it), each internal node the mass
+ * center of its 8 children in child order.
+ * hipcstone_set_mac replaces setMacGpu(exec, prefixes, numNodes, macSpheres, invTheta, box), source_center_gpu.h:
+ * 110-116 (source_center.hpp:127-140 with macs.hpp computeVecMacR2): centers[i][3] = (2 max(geoSize) inv_theta +
+ * |center - geoCenter|)^2 where the mass is nonzero, else 0.
+ * hipcstone_geo_mac_spheres replaces geoMacSpheresGpu, source_center_gpu.h:102-108 (computeMinMacR2): centers[i] =
+ * {geoCenter, (2 max(geoSize) inv_theta)^2}. Node keys are decoded as Hilbert keys (sfcKey -> SfcKind). ---- */
+#define HIPCSTONE_DECL_SOURCE_CENTERS(RS, T)                                                                           \
+    hipcstone_status hipcstone_leaf_mass_centers_##RS(hipcstone_stream s, const T* x, const T* y, const T* z,         \
+                                                      const T* m, const int32_t* leaf_to_internal,                    \
+                                                      int32_t num_leaves, const uint32_t* layout, T* centers);        \
+    hipcstone_status hipcstone_upsweep_centers_##RS(hipcstone_stream s, int32_t num_levels,                          \
+                                                    const int32_t* level_range_host, const int32_t* child_offsets,   \
+                                                    T* centers);
+HIPCSTONE_DECL_SOURCE_CENTERS(f32, float)
+HIPCSTONE_DECL_SOURCE_CENTERS(f64, double)
+#define HIPCSTONE_DECL_MACS(KS, K, RS, T)                                                                              \
+    hipcstone_status hipcstone_set_mac_##KS##_##RS(hipcstone_stream s, const K* prefixes, int32_t num_nodes,          \
+                                                   T* centers, float inv_theta, const hipcstone_box_##RS* box);       \
+    hipcstone_status hipcstone_geo_mac_spheres_##KS##_##RS(hipcstone_stream s, const K* prefixes, int32_t num_nodes,  \
+                                                           T* centers, float inv_theta,                               \
+                                                           const hipcstone_box_##RS* box);
+HIPCSTONE_DECL_MACS(u32, uint32_t, f32, float)
+HIPCSTONE_DECL_MACS(u32, uint32_t, f64, double)
+HIPCSTONE_DECL_MACS(u64, uint64_t, f32, float)
+HIPCSTONE_DECL_MACS(u64, uint64_t, f64, double)
 
 /* ---- a12: leaf layout. layout[0] = 0, layout[i+1] = sum(counts[0..i]) (the inclusive scan of
  * test/unit/neighbors/findneighbors.cpp:67-68 / focus/octree_focus_mpi.hpp:552-554). ---- */

@@ -75,6 +75,7 @@
 #define FLOOR floorf
 #define RINT rintf
 #define FABS fabsf
+#define SQRT sqrtf
 #define REAL_MAX FLT_MAX
 #include "oracle_real.inc"
 #undef REAL
@@ -82,6 +83,7 @@
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef SQRT
 #undef REAL_MAX
 
 #define REAL double
@@ -89,6 +91,7 @@
 #define FLOOR floor
 #define RINT rint
 #define FABS fabs
+#define SQRT sqrt
 #define REAL_MAX DBL_MAX
 #include "oracle_real.inc"
 #undef REAL
@@ -96,6 +99,7 @@
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef SQRT
 #undef REAL_MAX
 
 /* ---- key x coordinate templates ---- */
@@ -107,23 +111,27 @@
 #define FLOOR floorf
 #define RINT rintf
 #define FABS fabsf
+#define SQRT sqrtf
 #include "oracle_keyreal.inc"
 #undef REAL
 #undef RS
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef SQRT
 #define REAL double
 #define RS f64
 #define FLOOR floor
 #define RINT rint
 #define FABS fabs
+#define SQRT sqrt
 #include "oracle_keyreal.inc"
 #undef REAL
 #undef RS
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef SQRT
 #undef KEY
 #undef KS
 #undef MAXL
@@ -136,23 +144,27 @@
 #define FLOOR floorf
 #define RINT rintf
 #define FABS fabsf
+#define SQRT sqrtf
 #include "oracle_keyreal.inc"
 #undef REAL
 #undef RS
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef SQRT
 #define REAL double
 #define RS f64
 #define FLOOR floor
 #define RINT rint
 #define FABS fabs
+#define SQRT sqrt
 #include "oracle_keyreal.inc"
 #undef REAL
 #undef RS
 #undef FLOOR
 #undef RINT
 #undef FABS
+#undef SQRT
 #undef KEY
 #undef KS
 #undef MAXL

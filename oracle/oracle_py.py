@@ -187,6 +187,38 @@ class CpuLib:
         f(C.c_int(curve), _ptr(prefixes), C.c_size_t(N), _ptr(c), _ptr(s), C.byref(box))
         return c, s
 
+    # ---- source (mass) centers and MAC radii of the focus tree
+    def leaf_mass_centers(self, x, y, z, m, leaf_to_internal, layout, num_nodes, centers=None):
+        """centers[leafToInternal[leaf]] = (mass center, total |m|); other nodes keep `centers` (zeros if None)."""
+        T = x.dtype
+        c = np.zeros((num_nodes, 4), dtype=T) if centers is None else np.array(centers, dtype=T, copy=True)
+        l2i = np.ascontiguousarray(leaf_to_internal, dtype=np.int32)
+        self._f(f"leaf_mass_centers_{_RS[T]}")(_ptr(x), _ptr(y), _ptr(z), _ptr(np.ascontiguousarray(m, dtype=T)),
+                                               _ptr(l2i), C.c_int(l2i.size),
+                                               _ptr(np.ascontiguousarray(layout, dtype=np.uint32)), _ptr(c))
+        return c
+
+    def upsweep_centers(self, level_range, child_offsets, centers, num_levels):
+        c = np.array(centers, copy=True)
+        self._f(f"upsweep_centers_{_RS[c.dtype]}")(_ptr(np.ascontiguousarray(level_range, dtype=np.int32)),
+                                                    C.c_int(num_levels),
+                                                    _ptr(np.ascontiguousarray(child_offsets, dtype=np.int32)),
+                                                    _ptr(c))
+        return c
+
+    def set_mac(self, prefixes, centers, inv_theta, box):
+        c = np.array(centers, copy=True)
+        self._f(f"set_mac_{_KS[prefixes.dtype]}_{_RS[c.dtype]}")(_ptr(prefixes), C.c_size_t(prefixes.size), _ptr(c),
+                                                                   C.c_float(inv_theta), C.byref(box))
+        return c
+
+    def geo_mac_spheres(self, prefixes, real_dtype, inv_theta, box):
+        T = np.dtype(real_dtype)
+        c = np.zeros((prefixes.size, 4), dtype=T)
+        self._f(f"geo_mac_spheres_{_KS[prefixes.dtype]}_{_RS[T]}")(_ptr(prefixes), C.c_size_t(prefixes.size),
+                                                                     _ptr(c), C.c_float(inv_theta), C.byref(box))
+        return c
+
     def compute_bounding_box(self, x, y, z, h, layout, first, last, scale, centers):
         T = x.dtype
         c = np.array(centers, dtype=T, copy=True)

@@ -382,6 +382,39 @@ size_t haloChainImpl(int curve, T* x, T* y, T* z, T* h, size_t n, const BoxC<T>*
     return numLocal;
 }
 
+
+template<class T>
+void leafMassCentersImpl(const T* x, const T* y, const T* z, const T* m, const int32_t* leafToInternal,
+                         int32_t numLeaves, const uint32_t* layout, T* centers)
+{
+    size_t n = layout[numLeaves];
+    computeLeafMassCenter<T, T, T>(std::span<const T>(x, n), std::span<const T>(y, n), std::span<const T>(z, n),
+                                   std::span<const T>(m, n), std::span<const TreeNodeIndex>(leafToInternal, numLeaves),
+                                   layout, reinterpret_cast<SourceCenterType<T>*>(centers));
+}
+
+template<class T>
+void upsweepCentersImpl(const int32_t* levelRange, int numLevels, const int32_t* childOffsets, T* centers)
+{
+    upsweep(std::span<const TreeNodeIndex>(levelRange, size_t(numLevels) + 2), childOffsets,
+            reinterpret_cast<SourceCenterType<T>*>(centers), CombineSourceCenter<T>{});
+}
+
+template<class K, class T>
+void setMacImpl(const K* prefixes, size_t n, T* centers, float invTheta, const BoxC<T>* boxc)
+{
+    setMac<T, K>(std::span<const K>(prefixes, n), std::span<SourceCenterType<T>>(
+                                                       reinterpret_cast<SourceCenterType<T>*>(centers), n),
+                 invTheta, toBox(boxc));
+}
+
+template<class K, class T>
+void geoMacSpheresImpl(const K* prefixes, size_t n, T* centers, float invTheta, const BoxC<T>* boxc)
+{
+    geoMacSpheres<K, T>(std::span<const K>(prefixes, n), reinterpret_cast<SourceCenterType<T>*>(centers), invTheta,
+                        toBox(boxc));
+}
+
 } // namespace
 
 extern "C"
@@ -585,6 +618,15 @@ extern "C"
     {                                                                                                                  \
         return pipelineImpl<K, T>(curve, x, y, z, h, n, box, bucket, tree, numNodes, counts, cap, ngmax,              \
                                   nbStride, nb, nc, times);                                                            \
+    }                                                                                                                  \
+    void ref_set_mac_##KS##_##TS(const K* prefixes, size_t n, T* centers, float invTheta, const BoxC<T>* box)     \
+    {                                                                                                                  \
+        setMacImpl<K, T>(prefixes, n, centers, invTheta, box);                                                         \
+    }                                                                                                                  \
+    void ref_geo_mac_spheres_##KS##_##TS(const K* prefixes, size_t n, T* centers, float invTheta,                     \
+                                         const BoxC<T>* box)                                                           \
+    {                                                                                                                  \
+        geoMacSpheresImpl<K, T>(prefixes, n, centers, invTheta, box);                                                  \
     }
 
     REF_KR(uint32_t, u32, float, f32)
@@ -614,6 +656,15 @@ extern "C"
     {                                                                                                                  \
         ijloopExampleImpl<T>(x, y, z, h, m, first, last, box, childOffsets, parents, internalToLeaf, layout, centers,  \
                              sizes, ext, ngmax, rho, count, dmin, dmax, fx);                                           \
+    }                                                                                                                  \
+    void ref_leaf_mass_centers_##TS(const T* x, const T* y, const T* z, const T* m, const int32_t* leafToInternal,    \
+                                    int32_t numLeaves, const uint32_t* layout, T* centers)                             \
+    {                                                                                                                  \
+        leafMassCentersImpl<T>(x, y, z, m, leafToInternal, numLeaves, layout, centers);                               \
+    }                                                                                                                  \
+    void ref_upsweep_centers_##TS(const int32_t* levelRange, int numLevels, const int32_t* childOffsets, T* centers) \
+    {                                                                                                                  \
+        upsweepCentersImpl<T>(levelRange, numLevels, childOffsets, centers);                                           \
     }
 
     REF_T(float, f32)
