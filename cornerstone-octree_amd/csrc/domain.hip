@@ -412,6 +412,21 @@ struct DomainApi;
         {                                                                                                              \
             return hipcstone_compute_geo_centers_##KS##_##RS(s, pf, N, c, sz, b);                                     \
         }                                                                                                              \
+        static hipcstone_status leafCenters(hipcstone_stream s, const T* x, const T* y, const T* z, const T* m,       \
+                                            const int32_t* l2i, int32_t L, const uint32_t* layout, T* c)               \
+        {                                                                                                              \
+            return hipcstone_leaf_mass_centers_##RS(s, x, y, z, m, l2i, L, layout, c);                                \
+        }                                                                                                              \
+        static hipcstone_status upsweep(hipcstone_stream s, int32_t levels, const int32_t* lrh, const int32_t* co,    \
+                                        T* c)                                                                          \
+        {                                                                                                              \
+            return hipcstone_upsweep_centers_##RS(s, levels, lrh, co, c);                                             \
+        }                                                                                                              \
+        static hipcstone_status setMac(hipcstone_stream s, const K* pf, int32_t N, T* c, float invTheta,             \
+                                       const Box* b)                                                                   \
+        {                                                                                                              \
+            return hipcstone_set_mac_##KS##_##RS(s, pf, N, c, invTheta, b);                                           \
+        }                                                                                                              \
         static void makeBox(const T* lo, const T* hi, const uint8_t* bnd, Box* b)                                      \
         {                                                                                                              \
             hipcstone_make_box_##RS(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], bnd[0], bnd[1], bnd[2], b);             \
@@ -433,6 +448,10 @@ struct DomainBase
     virtual void view(hipcstone_domain_view* v) const = 0;
     virtual hipcstone_status gatherProperties(hipcstone_stream s, void* const* arrays, const size_t* elemBytes,
                                               int numArrays) = 0;
+    virtual hipcstone_status syncGrav(hipcstone_stream s, void* x, void* y, void* z, void* h, void* m, size_t n,
+                                      void* keysOut) = 0;
+    virtual hipcstone_status updateExpansionCenters(hipcstone_stream s, const void* x, const void* y, const void* z,
+                                                    const void* m) = 0;
 };
 
 template<class K, class T>
@@ -442,9 +461,10 @@ class Domain final : public DomainBase
     using Box = typename A::Box;
 
 public:
-    Domain(uint32_t bucket, uint32_t bucketFocus, const Box& box)
+    Domain(uint32_t bucket, uint32_t bucketFocus, float theta, const Box& box)
         : bucket_(bucket)
         , bucketFocus_(bucketFocus)
+        , invTheta_(float(1.0 / double(theta))) // the reference passes 1.0 / theta_ (a double) to a float argument
     {
         for (int d = 0; d < 3; ++d)
         {
@@ -500,7 +520,48 @@ public:
         focusConverged_ = conv;
         HC_STAGE(6, finish(s, x, y, z, h, keysOut));
 #undef HC_STAGE
-        first_ = false;
+        first_          = false;
+        centersCurrent_ = false;
+        return HIPCSTONE_OK;
+    }
+
+    // Domain::syncGrav at one rank (domain.hpp:220-287): the MAC flags decide nothing (no node lies outside the
+    // focus), so the particle order and both trees are those of sync; m is reordered with x, y, z, h, and the focus
+    // tree's expansion centers are the mass centers of its final tree (updateCenters, octree_focus_mpi.hpp:351-414)
+    // with the vector-MAC radius of 1 / theta (updateMacs -> setMacRadius, :446-453)
+    hipcstone_status syncGrav(hipcstone_stream s, void* x, void* y, void* z, void* h, void* m, size_t n,
+                              void* keysOut) override
+    {
+        if (!m) return invalid("domain syncGrav: null mass array");
+        if (auto e = sync(s, x, y, z, h, n, keysOut)) return e;
+        void* arrays[1]        = {m};
+        const size_t bytes[1] = {sizeof(T)};
+        if (auto e = gatherProperties(s, arrays, bytes, 1)) return e;
+        return updateExpansionCenters(s, x, y, z, m);
+    }
+
+    // Domain::updateExpansionCenters (domain.hpp:366-372): leaf mass centers over the focus layout, the upsweep, and
+    // setMacRadius(1 / theta); x, y, z, m are in the order of the last sync
+    hipcstone_status updateExpansionCenters(hipcstone_stream s, const void* xv, const void* yv, const void* zv,
+                                            const void* mv) override
+    {
+        if (n_ == 0) return invalid("domain expansion centers: no sync yet");
+        if (!xv || !yv || !zv || !mv) return invalid("domain expansion centers: null array");
+        if (auto e = expCenters_.reserve(size_t(fN_) * 4 * sizeof(T))) return e;
+        const int32_t NI = fN_ - fL_;
+        if (auto e = A::leafCenters(s, static_cast<const T*>(xv), static_cast<const T*>(yv),
+                                    static_cast<const T*>(zv), static_cast<const T*>(mv),
+                                    leafToInternal_.as<int32_t>() + NI, fL_, layout_.as<uint32_t>(),
+                                    expCenters_.as<T>()))
+            return e;
+        int32_t lr[32];
+        constexpr int32_t levels = int32_t(maxLevel<K>());
+        HC_CHECK(hipMemcpyAsync(lr, levelRange_.ptr, (levels + 2) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                stream(s)));
+        HC_CHECK(hipStreamSynchronize(stream(s)));
+        if (auto e = A::upsweep(s, levels, lr, childOffsets_.as<int32_t>(), expCenters_.as<T>())) return e;
+        if (auto e = A::setMac(s, prefixes_.as<K>(), fN_, expCenters_.as<T>(), invTheta_, &box_)) return e;
+        centersCurrent_ = true;
         return HIPCSTONE_OK;
     }
 
@@ -536,6 +597,7 @@ public:
         v->centers          = centers_.ptr;
         v->sizes            = sizes_.ptr;
         v->layout           = layout_.ptr;
+        v->expansion_centers = centersCurrent_ ? expCenters_.ptr : nullptr;
     }
 
     // reorder property arrays (n elements each, in the order the last sync received x, y, z, h) into the SFC order
@@ -895,6 +957,8 @@ private:
     }
 
     uint32_t bucket_, bucketFocus_;
+    float invTheta_;
+    bool centersCurrent_ = false;
     T lo_[3]{}, hi_[3]{};
     uint8_t bnd_[3]{};
     Box box_{};
@@ -912,6 +976,7 @@ private:
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_;
     DeviceBuf injKeyBuf_, injFlags_, injSortWs_, injScanWs_, propScratch_;
+    DeviceBuf expCenters_;
 };
 
 } // namespace hipcstone
@@ -929,22 +994,22 @@ extern "C"
 hipcstone_status hipcstone_domain_create(hipcstone_domain** d, int key_bits, int real_bits, uint32_t bucket_size,
                                          uint32_t bucket_size_focus, float theta, const void* box)
 {
-    (void)theta; // the multipole acceptance angle decides nothing at one rank
     if (!d || !box) return invalid("domain_create: null pointer");
+    if (!(theta > 0.f)) return invalid("domain_create: theta must be positive");
     if (bucket_size == 0 || bucket_size_focus == 0) return invalid("domain_create: bucket sizes must be positive");
     if (bucket_size_focus > bucket_size) return invalid("domain_create: bucket_size_focus above bucket_size");
     auto* h = new hipcstone_domain;
     if (key_bits == 64 && real_bits == 32)
-        h->impl = std::make_unique<Domain<uint64_t, float>>(bucket_size, bucket_size_focus,
+        h->impl = std::make_unique<Domain<uint64_t, float>>(bucket_size, bucket_size_focus, theta,
                                                            *static_cast<const hipcstone_box_f32*>(box));
     else if (key_bits == 64 && real_bits == 64)
-        h->impl = std::make_unique<Domain<uint64_t, double>>(bucket_size, bucket_size_focus,
+        h->impl = std::make_unique<Domain<uint64_t, double>>(bucket_size, bucket_size_focus, theta,
                                                             *static_cast<const hipcstone_box_f64*>(box));
     else if (key_bits == 32 && real_bits == 32)
-        h->impl = std::make_unique<Domain<uint32_t, float>>(bucket_size, bucket_size_focus,
+        h->impl = std::make_unique<Domain<uint32_t, float>>(bucket_size, bucket_size_focus, theta,
                                                            *static_cast<const hipcstone_box_f32*>(box));
     else if (key_bits == 32 && real_bits == 64)
-        h->impl = std::make_unique<Domain<uint32_t, double>>(bucket_size, bucket_size_focus,
+        h->impl = std::make_unique<Domain<uint32_t, double>>(bucket_size, bucket_size_focus, theta,
                                                             *static_cast<const hipcstone_box_f64*>(box));
     else
     {
@@ -980,6 +1045,20 @@ hipcstone_status hipcstone_domain_gather_properties(hipcstone_domain* d, hipcsto
 {
     if (!d) return invalid("domain_gather_properties: null handle");
     return d->impl->gatherProperties(s, arrays, elem_bytes, num_arrays);
+}
+
+hipcstone_status hipcstone_domain_sync_grav(hipcstone_domain* d, hipcstone_stream s, void* x, void* y, void* z,
+                                            void* h, void* m, size_t n, void* keys_out)
+{
+    if (!d) return invalid("domain_sync_grav: null handle");
+    return d->impl->syncGrav(s, x, y, z, h, m, n, keys_out);
+}
+
+hipcstone_status hipcstone_domain_update_expansion_centers(hipcstone_domain* d, hipcstone_stream s, const void* x,
+                                                           const void* y, const void* z, const void* m)
+{
+    if (!d) return invalid("domain_update_expansion_centers: null handle");
+    return d->impl->updateExpansionCenters(s, x, y, z, m);
 }
 
 hipcstone_status hipcstone_domain_set_timing(hipcstone_domain* d, int enable)

@@ -682,7 +682,8 @@ class DomainView(C.Structure):
         ("box_hi", C.c_double * 3),
     ] + [(name, C.c_void_p) for name in (
         "keys", "global_leaves", "global_counts", "leaves", "counts", "node_counts", "prefixes", "child_offsets",
-        "parents", "internal_to_leaf", "leaf_to_internal", "level_range", "centers", "sizes", "layout")]
+        "parents", "internal_to_leaf", "leaf_to_internal", "level_range", "centers", "sizes", "layout",
+        "expansion_centers")]
 
 
 class Domain:
@@ -705,6 +706,24 @@ class Domain:
         n = x.count if n is None else n
         _check(_lib.hipcstone_domain_sync(self.handle, _stream(stream), _p(x), _p(y), _p(z), _p(h), C.c_size_t(n),
                                           _p(keys_out)), "domain_sync")
+
+    def sync_grav(self, x, y, z, h, m, n=None, keys_out=None, stream=None):
+        """Domain::syncGrav: sync, m reordered with x, y, z, h, and the focus tree's expansion centers."""
+        n = x.count if n is None else n
+        _check(_lib.hipcstone_domain_sync_grav(self.handle, _stream(stream), _p(x), _p(y), _p(z), _p(h), _p(m),
+                                               C.c_size_t(n), _p(keys_out)), "domain_sync_grav")
+
+    def update_expansion_centers(self, x, y, z, m, stream=None):
+        """Domain::updateExpansionCenters for arrays in the order of the last sync."""
+        _check(_lib.hipcstone_domain_update_expansion_centers(self.handle, _stream(stream), _p(x), _p(y), _p(z),
+                                                              _p(m)), "domain_update_expansion_centers")
+
+    def expansion_centers(self):
+        """(num_nodes, 4) mass center x, y, z and squared MAC radius per focus node, or None."""
+        v = self.view()
+        if not v.expansion_centers:
+            return None
+        return download_ptr(v.expansion_centers, 4 * v.num_nodes, self.real_dtype).reshape(v.num_nodes, 4)
 
     def view(self) -> DomainView:
         v = DomainView()

@@ -523,7 +523,8 @@ hipcstone_status hipcstone_pipeline_stage_times(hipcstone_pipeline* p, double* m
  * (bucketSizeFocus) starts from the root with count bucketSizeFocus+1, is updated until converged on the first call
  * and once per later call (octree_focus_mpi.hpp:585-603, domain.hpp:200-203) with every global leaf key mandatory.
  * With one rank there are no halos: start_index = 0, end_index = n, and the particle layout is the scan of the focus
- * leaf counts. theta is accepted for interface parity; it decides nothing at one rank. box points to a
+ * leaf counts. theta sets the MAC radius of the expansion centers (1/theta); the MAC flags it also drives in the
+ * reference select nothing at one rank (every node lies in the focus). box points to a
  * hipcstone_box_f32/f64 matching real_bits. keys_out (optional, n keys) receives the sorted keys. ---- */
 typedef struct hipcstone_domain hipcstone_domain;
 
@@ -556,6 +557,9 @@ typedef struct
     void* centers; /* geometric node centers / sizes, Vec3 per node (T) */
     void* sizes;
     void* layout; /* first particle of each focus leaf, num_leaves + 1 (u32) */
+    /* focus-tree expansion centers, 4 T per node (mass center x, y, z, squared vector-MAC radius of 1/theta; 0 for a
+     * massless node), after hipcstone_domain_sync_grav / _update_expansion_centers on the current tree, else NULL */
+    void* expansion_centers;
 } hipcstone_domain_view;
 
 hipcstone_status hipcstone_domain_create(hipcstone_domain** d, int key_bits, int real_bits, uint32_t bucket_size,
@@ -570,6 +574,18 @@ hipcstone_status hipcstone_domain_get_view(const hipcstone_domain* d, hipcstone_
  * x, y, z, h) is reordered in place into that sync's SFC order. ---- */
 hipcstone_status hipcstone_domain_gather_properties(hipcstone_domain* d, hipcstone_stream s, void* const* arrays,
                                                     const size_t* elem_bytes, int num_arrays);
+/* Domain::syncGrav at one rank. Replaces syncGrav(keys, x, y, z, h, m, {}, scratch), domain/domain.hpp:220-287: the
+ * particle order and both trees are those of hipcstone_domain_sync (its MAC-driven refinement selects nothing at one
+ * rank), m (n T) is reordered with x, y, z, h, and the focus tree's expansion centers are computed as by
+ * hipcstone_domain_update_expansion_centers. ---- */
+hipcstone_status hipcstone_domain_sync_grav(hipcstone_domain* d, hipcstone_stream s, void* x, void* y, void* z,
+                                            void* h, void* m, size_t n, void* keys_out);
+/* Replaces Domain::updateExpansionCenters(x, y, z, m, scratch), domain/domain.hpp:366-372 (FocusedOctree::updateCenters
+ * + setMacRadius(1 / theta), focus/octree_focus_mpi.hpp:351-414, :446-453): mass centers of the focus leaves over the
+ * domain's layout (the CPU path's summation order), the upsweep, and the squared vector-MAC radius; x, y, z, m are
+ * device arrays in the order of the last sync. The result is the view's expansion_centers. ---- */
+hipcstone_status hipcstone_domain_update_expansion_centers(hipcstone_domain* d, hipcstone_stream s, const void* x,
+                                                           const void* y, const void* z, const void* m);
 /* per-stage timing as for the pipeline; stages: 0 box, 1 keys, 2 sort+gather, 3 global tree, 4 focus update,
  * 5 focus counts, 6 centers + layout + copy-out */
 #define HIPCSTONE_DOMAIN_STAGES 7

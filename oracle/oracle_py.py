@@ -608,3 +608,26 @@ class CpuDomain:
         return dict(keys=keys, perm=perm, x=xs, y=ys, z=zs, h=hs, lo=self.lo, hi=self.hi, gtree=self.gtree,
                     gcounts=self.gcounts, gsteps=gsteps, fleaves=self.fleaves, fcounts=self.fleafcounts,
                     fnodecounts=self.fnodecounts, fsteps=fsteps, converged=conv)
+
+    def sync_grav(self, x, y, z, h, m, theta):
+        """Domain::syncGrav at one rank (domain.hpp:220-287): the MAC-driven refinement selects nothing (every node
+        lies in the focus), so the order and trees are sync's; m is reordered with x, y, z, h and the expansion
+        centers are those of expansion_centers() on the final focus tree."""
+        r = self.sync(x, y, z, h)
+        r["m"] = m[r["perm"]]
+        r["expansion_centers"], r["octree"] = self.expansion_centers(r["x"], r["y"], r["z"], r["m"], theta)
+        return r
+
+    def expansion_centers(self, x, y, z, m, theta):
+        """Domain::updateExpansionCenters (domain.hpp:366-372): the focus tree's leaf mass centers over the layout of
+        its leaf counts (updateCenters, octree_focus_mpi.hpp:351-414), the upsweep, and the squared vector-MAC radius
+        of 1.0 / theta (a double, passed as float to setMacRadius, :446-453); returns (centers, octree)."""
+        o = self.lib.build_octree(self.fleaves)
+        L = self.fleaves.size - 1
+        n_int = o["prefixes"].size - L
+        layout = np.zeros(L + 1, dtype=np.uint32)
+        layout[1:] = np.cumsum(self.fleafcounts, dtype=np.uint64).astype(np.uint32)
+        c = self.lib.leaf_mass_centers(x, y, z, m, o["leafToInternal"][n_int:], layout, o["prefixes"].size)
+        c = self.lib.upsweep_centers(o["levelRange"], o["childOffsets"], c, o["levelRange"].size - 2)
+        inv_theta = np.float32(1.0 / np.float64(np.float32(theta)))
+        return self.lib.set_mac(o["prefixes"], c, float(inv_theta), self.box()), o

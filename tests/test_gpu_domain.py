@@ -124,3 +124,49 @@ def test_domain_gathers_properties_in_sync_order(hip, oracle):
     assert np.array_equal(props[1].download(), (ids * 3).astype(np.uint64)[perm])
     assert np.array_equal(props[2].download(), ids.astype(np.uint8)[perm])
     assert np.array_equal(props[3].download(), ids.astype(np.uint16)[perm])
+
+
+@pytest.mark.parametrize("kd,T,bucket,bucket_focus,theta", [(np.uint64, np.float32, 64, 16, 0.5),
+                                                            (np.uint64, np.float64, 32, 8, 0.7),
+                                                            (np.uint32, np.float32, 64, 32, 0.6)])
+def test_domain_sync_grav_steps(hip, oracle, kd, T, bucket, bucket_focus, theta):
+    """Domain::syncGrav at one rank: sync's order and trees, m reordered with x, y, z, h, and the focus tree's
+    expansion centers (mass centers + squared vector-MAC radius of 1/theta), bit for bit, over several steps; then
+    updateExpansionCenters on moved-in-place coordinates."""
+    n = 90000
+    x, y, z, h = plummer_input(n, T, seed=35)
+    rng = np.random.default_rng(36)
+    m = (rng.random(n) * 2 - 0.4).astype(T)
+    m[rng.random(n) < 0.03] = 0
+    gbox = hip.make_box(-1.0, 1.0, T)
+    dom = hip.Domain(gbox, key_bits=64 if kd == np.uint64 else 32, bucket_size=bucket,
+                     bucket_size_focus=bucket_focus, theta=theta)
+    ref = CpuDomain(oracle, kd, T, bucket, bucket_focus, -1.0, 1.0)
+    d = [dev(hip, a) for a in (x, y, z, h, m)]
+    for step in range(3):
+        print(f"syncGrav step {step}: {np.dtype(kd).name} {np.dtype(T).name} bucket {bucket}/{bucket_focus}",
+              flush=True)
+        host = [a.download() for a in d]
+        o = ref.sync_grav(*host, theta)
+        dom.sync_grav(*d)
+        for name, a in zip("xyzhm", d):
+            assert np.array_equal(a.download(), o[name]), name
+        g = dom.arrays()
+        assert np.array_equal(g["leaves"], o["fleaves"]) and np.array_equal(g["prefixes"], o["octree"]["prefixes"])
+        ec = dom.expansion_centers()
+        assert ec is not None and np.array_equal(ec.view(np.uint8), o["expansion_centers"].view(np.uint8))
+        assert ec[0, 3] > 0 and (ec[:, 3] == 0).any()
+        moved = [(a.download() + rng.normal(0, 0.004, n)).astype(T) for a in d[:3]]
+        for a, mv in zip(d[:3], moved):
+            a.upload(np.clip(mv, -1, 1).astype(T))
+    # plain sync clears the centers; updateExpansionCenters recomputes them for the arrays given
+    host = [a.download() for a in d]
+    o = ref.sync(*host[:4])
+    dom.sync(*d[:4])
+    assert dom.expansion_centers() is None
+    dom.gather_properties([d[4]])
+    ms = host[4][o["perm"]]
+    assert np.array_equal(d[4].download(), ms)
+    dom.update_expansion_centers(d[0], d[1], d[2], d[4])
+    ref_c, _ = ref.expansion_centers(o["x"], o["y"], o["z"], ms, theta)
+    assert np.array_equal(dom.expansion_centers().view(np.uint8), ref_c.view(np.uint8))
