@@ -279,6 +279,90 @@ def domain_bench(args):
     print(json.dumps(result), flush=True)
 
 
+def grav_bench(args):
+    """One-rank Domain::syncGrav: the same 16,777,216 fp32 Plummer particles with equal masses 1/n, Hilbert-64 keys,
+    bucketSize 64 for the global leaf array and 16 for the focus tree, theta 0.5, box fitted to the particles; one
+    step = displacement by 1e-3 h + Domain::syncGrav (sync, masses reordered with the particles, focus-tree expansion
+    centers: leaf mass centers, upsweep, vector-MAC radii)."""
+    import hipcstone as hip
+    from hipcstone import inputs
+
+    n, K, W = args.n, args.steps, args.warmup
+    T = np.float32
+    a = 0.05
+    x, y, z, r = inputs.plummer(n, a, seed=2024, dtype=T)
+    h = inputs.plummer_h(r, n, a, 100.0, dtype=T)
+    m = np.full(n, 1.0 / n, dtype=T)
+    box = hip.make_box(-1.0, 1.0, T)
+    dom = hip.Domain(box, key_bits=64, bucket_size=64, bucket_size_focus=16, theta=0.5)
+    d = [hip.DeviceArray.from_numpy(v) for v in (x, y, z, h, m)]
+
+    def step(k):
+        hip.displace(d[0], d[1], d[2], d[3], n, 1e-3, 7, k, box)
+        dom.sync_grav(*d, n)
+
+    step(0)
+    for k in range(W):
+        step(1 + k)
+    hip.device_sync()
+    events = [hip.Event() for _ in range(K + 1)]
+    dom.set_timing(True)
+    events[0].record()
+    for k in range(K):
+        step(1 + W + k)
+        events[k + 1].record()
+    hip.device_sync()
+    per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
+    total_ms = float(per_step.sum())
+    stages = dom.stage_times()
+    dom.set_timing(False)
+    v = dom.view()
+    result = {
+        "metric": "Mparticles/s one-rank Domain::syncGrav (16M fp32)",
+        "value": round(n * K / (total_ms / 1e3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(total_ms / K, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": ("one-rank Domain::syncGrav: 16,777,216 fp32 Plummer particles (a=0.05), masses 1/n, fitted "
+                         "box, Hilbert-64 keys, bucketSize 64 / focus 16, theta 0.5, 1e-3 h displacement per step"),
+            "particles": n, "global_leaves": int(v.num_global_leaves), "focus_leaves": int(v.num_leaves),
+            "focus_nodes": int(v.num_nodes),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
+                       "p99": round(float(np.percentile(per_step, 99)), 3)},
+        "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()
+                   if c > 0},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_stage_baseline"] = grav_cpu_stage_baseline(dom, d)
+    print(json.dumps(result), flush=True)
+
+
+def grav_cpu_stage_baseline(dom, d):
+    """The reference's CPU expansion-center functions (computeLeafMassCenter, upsweep with CombineSourceCenter,
+    setMac, through oracle/_ref) timed on the GPU domain's last focus tree and particles, beside the GPU stage."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_py import REF, make_box
+
+    if REF is None:
+        return None
+    g = dom.arrays()
+    x, y, z, m = (a.download() for a in (d[0], d[1], d[2], d[4]))
+    N, L = g["prefixes"].size, g["layout"].size - 1
+    l2i = g["leafToInternal"][N - L:]
+    box = make_box(g["box_lo"], g["box_hi"], np.float32)
+    t0 = time.perf_counter()
+    c = REF.leaf_mass_centers(x, y, z, m, l2i, g["layout"], N)
+    c = REF.upsweep_centers(g["levelRange"], g["childOffsets"], c, g["levelRange"].size - 2)
+    c = REF.set_mac(g["prefixes"], c, float(np.float32(1.0 / np.float64(np.float32(0.5)))), box)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    ok = bool(np.array_equal(c.view(np.uint8), dom.expansion_centers().view(np.uint8)))
+    return {"stage": "expansion_centers", "cpu_ms": round(cpu_ms, 2), "cores": REF.omp_threads(),
+            "kind": "reference", "matches_gpu": ok}
+
+
 def ijloop_bench(args):
     """ijloop over the plain pipeline's tree: the same 16,777,216 fp32 Plummer particles (h from the analytic
     density), Hilbert-64 keys, bucketSize 64, a converged tree; one step = one ijLoop of the example interaction
@@ -375,9 +459,10 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "halo", "domain", "ijloop"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "halo", "domain", "grav", "ijloop"], default="plain",
                     help="plain: the 16.8M neighbor pipeline (default); halo: the 134M rank-share chain; "
-                         "domain: one-rank Domain::sync + neighbors at 16.8M; ijloop: the ijLoop example at 16.8M")
+                         "domain: one-rank Domain::sync + neighbors at 16.8M; grav: one-rank Domain::syncGrav at "
+                         "16.8M; ijloop: the ijLoop example at 16.8M")
     ap.add_argument("--n-halo", type=int, default=134217728)
     ap.add_argument("--halo-cpu-sample", type=int, default=8388608,
                     help="particles of the CPU baseline sample of the halo workload")
@@ -414,6 +499,9 @@ def main():
         return ijloop_bench(args)
     if args.workload == "domain":
         domain_bench(args)
+        return
+    if args.workload == "grav":
+        grav_bench(args)
         return
 
     import hipcstone as hip

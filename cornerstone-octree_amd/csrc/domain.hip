@@ -548,6 +548,7 @@ public:
         if (n_ == 0) return invalid("domain expansion centers: no sync yet");
         if (!xv || !yv || !zv || !mv) return invalid("domain expansion centers: null array");
         if (auto e = expCenters_.reserve(size_t(fN_) * 4 * sizeof(T))) return e;
+        if (auto e = this->timer.begin(7, s)) return e;
         const int32_t NI = fN_ - fL_;
         if (auto e = A::leafCenters(s, static_cast<const T*>(xv), static_cast<const T*>(yv),
                                     static_cast<const T*>(zv), static_cast<const T*>(mv),
@@ -561,6 +562,7 @@ public:
         HC_CHECK(hipStreamSynchronize(stream(s)));
         if (auto e = A::upsweep(s, levels, lr, childOffsets_.as<int32_t>(), expCenters_.as<T>())) return e;
         if (auto e = A::setMac(s, prefixes_.as<K>(), fN_, expCenters_.as<T>(), invTheta_, &box_)) return e;
+        if (auto e = this->timer.end(s)) return e;
         centersCurrent_ = true;
         return HIPCSTONE_OK;
     }

@@ -690,7 +690,8 @@ class Domain:
     """One-rank Domain::sync (hipcstone_domain_*): fitted box, Hilbert keys, sort, global leaf array, focus tree;
     sync reorders the particle arrays in place into SFC order."""
 
-    STAGES = ("box", "keys", "sort_gather", "global_tree", "focus_update", "focus_counts", "finish")
+    STAGES = ("box", "keys", "sort_gather", "global_tree", "focus_update", "focus_counts", "finish",
+              "expansion_centers")
 
     def __init__(self, box, key_bits=64, bucket_size=64, bucket_size_focus=16, theta=0.5):
         self.handle = C.c_void_p()

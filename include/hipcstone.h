@@ -587,8 +587,8 @@ hipcstone_status hipcstone_domain_sync_grav(hipcstone_domain* d, hipcstone_strea
 hipcstone_status hipcstone_domain_update_expansion_centers(hipcstone_domain* d, hipcstone_stream s, const void* x,
                                                            const void* y, const void* z, const void* m);
 /* per-stage timing as for the pipeline; stages: 0 box, 1 keys, 2 sort+gather, 3 global tree, 4 focus update,
- * 5 focus counts, 6 centers + layout + copy-out */
-#define HIPCSTONE_DOMAIN_STAGES 7
+ * 5 focus counts, 6 centers + layout + copy-out, 7 expansion centers */
+#define HIPCSTONE_DOMAIN_STAGES 8
 hipcstone_status hipcstone_domain_set_timing(hipcstone_domain* d, int enable);
 hipcstone_status hipcstone_domain_stage_times(hipcstone_domain* d, double* ms, int32_t* launches);
 
