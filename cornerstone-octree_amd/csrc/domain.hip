@@ -207,14 +207,18 @@ __global__ void enforceKeysKernel(const K* __restrict__ keys, int32_t numKeys, c
             if (parent == 0) break;
         }
     }
+    // ops and status only grow during this kernel (compare-and-swap 0 -> 1, atomic max), so an atomic max whose
+    // value the target already holds is skipped: on a root-only focus tree every key would otherwise hit ops[0]
     if (trySplit)
     {
         int levelDiff = lastNzPlace(key) - int(levelHave);
         st            = levelDiff > 1 ? 3 : 2;
         levelDiff     = levelDiff < 1 ? levelDiff : 1;
-        atomicMax(ops + nodeIdx, 1 << (3 * levelDiff));
+        const int v   = 1 << (3 * levelDiff);
+        if (__hip_atomic_load(ops + nodeIdx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v)
+            atomicMax(ops + nodeIdx, v);
     }
-    if (st) atomicMax(status, st);
+    if (st && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < st) atomicMax(status, st);
 }
 
 // focus/rebalance.hpp:90-107 nzAncestorOp into a separate array (the reference's in-place parallel update reads
