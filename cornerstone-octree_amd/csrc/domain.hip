@@ -73,17 +73,25 @@ __global__ __launch_bounds__(kMmThreads) void minMaxPartialKernel(const T* __res
 }
 
 template<class T>
-__global__ void minMaxFinalKernel(const T* __restrict__ partials, int numPartials, T* __restrict__ out)
+__global__ __launch_bounds__(6 * kWave) void minMaxFinalKernel(const T* __restrict__ partials, int numPartials,
+                                                                T* __restrict__ out)
 {
-    int c = threadIdx.x;
-    if (c >= 6) return;
+    // wave c reduces component c (xmin, xmax, ymin, ...) of the per-block partials: strided per lane, then across
+    // the wave, as the partials themselves were reduced
+    const int c = threadIdx.x / kWave, lane = laneId();
+    const bool isMax = c & 1;
     T v = partials[c];
-    for (int b = 1; b < numPartials; ++b)
+    for (int b = lane; b < numPartials; b += kWave)
     {
         T o = partials[size_t(b) * 6 + c];
-        v   = (c & 1) ? (o > v ? o : v) : (o < v ? o : v);
+        v   = isMax ? (o > v ? o : v) : (o < v ? o : v);
     }
-    out[c] = v;
+    for (int off = kWave / 2; off > 0; off >>= 1)
+    {
+        T o = __shfl_xor(v, off, kWave);
+        v   = isMax ? (o > v ? o : v) : (o < v ? o : v);
+    }
+    if (lane == 0) out[c] = v;
 }
 
 // ---------------------------------------------------------------- global tree: largest leaf count
@@ -223,7 +231,8 @@ __global__ void enforceKeysKernel(const K* __restrict__ keys, int32_t numKeys, c
 
 // focus/rebalance.hpp:90-107 nzAncestorOp into a separate array (the reference's in-place parallel update reads
 // the same values: a rewritten ancestor only ever takes the op of its own closest non-zero ancestor when it is that
-// ancestor's left-most descendant, which leaves every descendant's answer unchanged); changes counts ops != 1
+// ancestor's left-most descendant, which leaves every descendant's answer unchanged); changes is set to 1 if
+// some op differs from 1
 template<class K>
 __global__ void protectAncestorsKernel(const K* __restrict__ prefixes, const int32_t* __restrict__ parents,
                                        const int32_t* __restrict__ in, int32_t numNodes, int32_t* __restrict__ out,
@@ -245,12 +254,15 @@ __global__ void protectAncestorsKernel(const K* __restrict__ prefixes, const int
         out[i] = op;
         notOne = op != 1;
     }
-    uint64_t b = ballot(notOne);
-    if (laneId() == 0 && b) atomicAdd(changes, int32_t(__builtin_popcountll(b)));
+    // only whether some op differs from 1 is used: set the flag once instead of counting (one atomic per wave on a
+    // single address serializes over millions of nodes)
+    const uint64_t b = ballot(notOne);
+    if (laneId() == 0 && b && __hip_atomic_load(changes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __hip_atomic_store(changes, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void gatherLeafOpsKernel(const int32_t* __restrict__ nodeOps, const int32_t* __restrict__ leafToInternal,
-                                    int32_t numLeaves, int32_t* __restrict__ leafOps, int32_t* notOneCount)
+                                    int32_t numLeaves, int32_t* __restrict__ leafOps, int32_t* anyNotOne)
 {
     int32_t l   = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
     bool notOne = false;
@@ -261,8 +273,9 @@ __global__ void gatherLeafOpsKernel(const int32_t* __restrict__ nodeOps, const i
         notOne     = op != 1;
     }
     else if (l == numLeaves) { leafOps[l] = 0; }
-    uint64_t b = ballot(notOne);
-    if (laneId() == 0 && b) atomicAdd(notOneCount, int32_t(__builtin_popcountll(b)));
+    const uint64_t b = ballot(notOne);
+    if (laneId() == 0 && b && __hip_atomic_load(anyNotOne, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __hip_atomic_store(anyNotOne, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // node counts of a linked tree from the sorted particle keys: the keys in [start, start + range) of each node, as
@@ -650,7 +663,7 @@ private:
             int blocks = int(std::min<size_t>(kMmBlocks, ceilDiv(n_, kMmThreads)));
             minMaxPartialKernel<T><<<blocks, kMmThreads, 0, stream(s)>>>(x, y, z, n_, mmPartials_.as<T>());
             HC_CHECK_LAUNCH("minMaxPartialKernel");
-            minMaxFinalKernel<T><<<1, 64, 0, stream(s)>>>(mmPartials_.as<T>(), blocks, mmOut_.as<T>());
+            minMaxFinalKernel<T><<<1, 6 * kWave, 0, stream(s)>>>(mmPartials_.as<T>(), blocks, mmOut_.as<T>());
             HC_CHECK_LAUNCH("minMaxFinalKernel");
             T mm[6];
             HC_CHECK(hipMemcpyAsync(mm, mmOut_.ptr, sizeof(mm), hipMemcpyDeviceToHost, stream(s)));
@@ -853,7 +866,7 @@ private:
     {
         const int32_t L = fL_, N = fN_, NI = fN_ - fL_;
         const int32_t M = numGlobal_ + 1;
-        int32_t* flags  = small_.as<int32_t>(); // [0] status, [1] protect changes, [2] leaf ops != 1
+        int32_t* flags  = small_.as<int32_t>(); // [0] status, [1] some protected op != 1, [2] some leaf op != 1
         zeroI32Kernel<<<1, 64, 0, stream(s)>>>(flags, 3);
         HC_CHECK_LAUNCH("zeroI32Kernel");
         focusNodeOpsKernel<K><<<ceilDiv(size_t(N), 256), 256, 0, stream(s)>>>(
