@@ -106,7 +106,8 @@ __global__ void maxCountKernel(const uint32_t* __restrict__ counts, int32_t n, u
         uint32_t o = __shfl_xor(m, off, kWave);
         m          = o > m ? o : m;
     }
-    if (laneId() == 0) atomicMax(out, m);
+    // the maximum only grows: skip the atomic when it already holds at least m (thousands of waves share *out)
+    if (laneId() == 0 && m > __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(out, m);
 }
 
 // ---------------------------------------------------------------- focus tree decisions

@@ -299,7 +299,10 @@ __global__ __launch_bounds__(256) void haloLeafBoundsKernel(HaloArgs<K, T> A, T*
         T c[3], s[3];
         load3(A.nodeCenters, 0, c);
         load3(A.nodeSizes, 0, s);
-        if (overlapBox(c, s, tC, tS, A.box)) atomicMin(&ctr->rootWitness, i);
+        // the witness only decreases: skip the atomic when it already holds a lower leaf
+        if (overlapBox(c, s, tC, tS, A.box) &&
+            i < __hip_atomic_load(&ctr->rootWitness, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&ctr->rootWitness, i);
     }
 }
 
