@@ -59,6 +59,12 @@ struct Api;
         {                                                                                                              \
             return hipcstone_sort_gather_##KS##_##RS(s, k, p, n, xi, yi, zi, hi, xo, yo, zo, ho, kb, vb, ws, wsb);    \
         }                                                                                                              \
+        static size_t sortByKeyWs(size_t n) { return hipcstone_sort_by_key_workspace_size_##KS(n); }                  \
+        static hipcstone_status sortByKey(hipcstone_stream s, K* k, uint32_t* v, size_t n, K* kb, uint32_t* vb,       \
+                                          void* ws, size_t wsb)                                                        \
+        {                                                                                                              \
+            return hipcstone_sort_by_key_##KS(s, k, v, n, kb, vb, ws, wsb, int(8 * sizeof(K)));                       \
+        }                                                                                                              \
         static size_t buildWs(int32_t L) { return hipcstone_build_octree_workspace_size_##KS(L); }                    \
         static hipcstone_status update(hipcstone_stream s, const K* keys, size_t n, uint32_t bucket, const K* tree,   \
                                        const uint32_t* counts, int32_t* nn, K* nt, uint32_t* nc, int32_t cap,          \
@@ -111,6 +117,89 @@ HC_API(u32, uint32_t, f32, float)
 HC_API(u32, uint32_t, f64, double)
 HC_API(u64, uint64_t, f32, float)
 HC_API(u64, uint64_t, f64, double)
+
+// ---- direct construction of the converged leaf array (computeOctree, tree/csarray.hpp:429-440).
+// The reference iterates updateOctree from the root until the input tree is converged. Its fixpoint is unique: a node
+// is internal exactly when it holds more than bucketSize keys and lies above the deepest level (a node at or below
+// the bucket size cannot be internal: its deepest internal descendant would have 8 leaf children whose total fits the
+// bucket, which the rebalance merges; a node above it cannot be a leaf: the rebalance splits it). So the tree is built
+// top down, one level per launch: every internal node of a level splits into its 8 children, whose key ranges come
+// from binary searches inside the parent's range of the sorted keys; children above the bucket size (and above the
+// deepest level) form the next level, the others are leaves. The leaves are then sorted by key, and the counts are the
+// keys in each leaf's range, as computeNodeCounts gives them.
+template<class K>
+struct SplitNode
+{
+    K start;     // first key of the node
+    uint32_t lo; // its range of the sorted keys
+    uint32_t hi;
+};
+
+template<class K>
+__device__ inline uint32_t lowerBoundIn(const K* __restrict__ keys, uint32_t lo, uint32_t hi, K v)
+{
+    while (lo < hi)
+    {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// the root's key range: keys at or past nodeRange(0) (removed particles) belong to no leaf
+template<class K>
+__global__ void splitRootKernel(const K* __restrict__ keys, uint32_t n, SplitNode<K>* __restrict__ frontier,
+                                uint32_t* __restrict__ rootCount)
+{
+    uint32_t hi = lowerBoundIn(keys, 0u, n, nodeRange<K>(0));
+    frontier[0] = SplitNode<K>{K(0), 0u, hi};
+    *rootCount  = hi;
+}
+
+// 8 lanes per frontier node, lane k for child k; counters[0] counts the next frontier, counters[1] all leaves
+template<class K>
+__global__ __launch_bounds__(256) void splitLevelKernel(const K* __restrict__ keys, const SplitNode<K>* __restrict__ in,
+                                                        uint32_t nIn, unsigned childLevel, uint32_t bucket,
+                                                        SplitNode<K>* __restrict__ out, K* __restrict__ leafKeys,
+                                                        uint32_t* __restrict__ leafCounts, uint32_t* counters)
+{
+    const uint32_t gid  = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t node = gid >> 3;
+    const unsigned k    = gid & 7u;
+    const bool valid    = node < nIn;
+    SplitNode<K> f{K(0), 0u, 0u};
+    if (valid) f = in[node];
+    const K start = f.start + K(k) * nodeRange<K>(childLevel);
+    uint32_t b    = (valid && k > 0) ? lowerBoundIn(keys, f.lo, f.hi, start) : f.lo;
+    uint32_t e    = __shfl_down(b, 1, 8);
+    if (k == 7) e = f.hi;
+    const uint32_t cnt  = e - b;
+    const bool internal = valid && cnt > bucket && childLevel < maxLevel<K>();
+    const bool leaf     = valid && !internal;
+    const uint64_t mi   = ballot(internal);
+    const uint64_t ml   = ballot(leaf);
+    uint32_t bi = 0, bl = 0;
+    if (laneId() == 0)
+    {
+        if (mi) bi = atomicAdd(counters, uint32_t(__builtin_popcountll(mi)));
+        if (ml) bl = atomicAdd(counters + 1, uint32_t(__builtin_popcountll(ml)));
+    }
+    bi = uniform(bi);
+    bl = uniform(bl);
+    if (internal) out[bi + laneRank(mi)] = SplitNode<K>{start, b, e};
+    if (leaf)
+    {
+        leafKeys[bl + laneRank(ml)]   = start;
+        leafCounts[bl + laneRank(ml)] = cnt;
+    }
+}
+
+__global__ void setU32PairKernel(uint32_t* p, uint32_t a, uint32_t b)
+{
+    p[0] = a;
+    p[1] = b;
+}
 
 template<class K, class T>
 class Pipeline final : public PipelineBase
@@ -357,23 +446,83 @@ private:
         return invalid("pipeline: leaf capacity growth did not converge");
     }
 
-    // computeOctree: root {0, nodeRange(0)} with counts {n}, update until the input tree was converged
+    // computeOctree: the converged leaf array of the sorted keys, built level by level (splitLevelKernel)
     hipcstone_status computeOctree(hipcstone_stream s)
     {
+        numLeaves_ = 0; // nothing to keep across buffer growth
         if (auto e = reserveLeaves(std::max<int32_t>(capLeaves_, 4096))) return e;
-        K root[2]     = {K(0), nodeRange<K>(0)};
-        uint32_t cnt  = uint32_t(n_);
-        HC_CHECK(hipMemcpyAsync(tree_[treeCur_].ptr, root, sizeof(root), hipMemcpyHostToDevice, stream(s)));
-        HC_CHECK(hipMemcpyAsync(counts_[treeCur_].ptr, &cnt, sizeof(cnt), hipMemcpyHostToDevice, stream(s)));
-        numLeaves_  = 1;
-        int calls   = 0;
-        int conv    = 0;
-        while (!conv)
+        if (auto e = splitCounters_.reserve(2 * sizeof(uint32_t))) return e;
+        if (auto e = frontier_[0].reserve(sizeof(SplitNode<K>))) return e;
+        splitRootKernel<K><<<1, 1, 0, stream(s)>>>(keys_.as<K>(), uint32_t(n_), frontier_[0].as<SplitNode<K>>(),
+                                                   splitCounters_.as<uint32_t>());
+        HC_CHECK_LAUNCH("splitRootKernel");
+        uint32_t host[2];
+        HC_CHECK(hipMemcpyAsync(host, splitCounters_.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, stream(s)));
+        HC_CHECK(hipStreamSynchronize(stream(s)));
+        const K rootKeys[2] = {K(0), nodeRange<K>(0)};
+        if (host[0] <= bucket_)
         {
-            if (auto e = updateOnce(s, &conv)) return e;
-            if (++calls > 64) return invalid("pipeline: computeOctree did not converge in 64 steps");
+            HC_CHECK(hipMemcpyAsync(tree_[treeCur_].ptr, rootKeys, sizeof(rootKeys), hipMemcpyHostToDevice,
+                                    stream(s)));
+            HC_CHECK(hipMemcpyAsync(counts_[treeCur_].ptr, &host[0], sizeof(uint32_t), hipMemcpyHostToDevice,
+                                    stream(s)));
+            numLeaves_      = 1;
+            convergedCalls_ = 0;
+            return HIPCSTONE_OK;
         }
-        convergedCalls_ = calls;
+        uint32_t nIn = 1, leaves = 0;
+        int cur = 0, levels = 0;
+        for (unsigned childLevel = 1; nIn > 0; ++childLevel)
+        {
+            if (childLevel > maxLevel<K>()) return invalid("pipeline: computeOctree split below the deepest level");
+            const size_t nOutMax = size_t(nIn) * 8;
+            if (auto e = frontier_[1 - cur].reserve(nOutMax * sizeof(SplitNode<K>))) return e;
+            if (auto e = growKeep(splitKeys_, (leaves + nOutMax) * sizeof(K), leaves * sizeof(K), s)) return e;
+            if (auto e = growKeep(splitCounts_, (leaves + nOutMax) * sizeof(uint32_t), leaves * sizeof(uint32_t), s))
+                return e;
+            setU32PairKernel<<<1, 1, 0, stream(s)>>>(splitCounters_.as<uint32_t>(), 0u, leaves);
+            HC_CHECK_LAUNCH("setU32PairKernel");
+            splitLevelKernel<K><<<ceilDiv(nOutMax, 256), 256, 0, stream(s)>>>(
+                keys_.as<K>(), frontier_[cur].as<SplitNode<K>>(), nIn, childLevel, bucket_,
+                frontier_[1 - cur].as<SplitNode<K>>(), splitKeys_.as<K>(), splitCounts_.as<uint32_t>(),
+                splitCounters_.as<uint32_t>());
+            HC_CHECK_LAUNCH("splitLevelKernel");
+            HC_CHECK(hipMemcpyAsync(host, splitCounters_.ptr, sizeof(host), hipMemcpyDeviceToHost, stream(s)));
+            HC_CHECK(hipStreamSynchronize(stream(s)));
+            nIn    = host[0];
+            leaves = host[1];
+            cur    = 1 - cur;
+            ++levels;
+        }
+        if (leaves > uint32_t(INT32_MAX - 1)) return invalid("pipeline: computeOctree leaf count above 2^31");
+        const int32_t L = int32_t(leaves);
+        if (auto e = reserveLeaves(L)) return e;
+        K* tree          = tree_[treeCur_].template as<K>();
+        uint32_t* counts = counts_[treeCur_].template as<uint32_t>();
+        HC_CHECK(hipMemcpyAsync(tree, splitKeys_.ptr, size_t(L) * sizeof(K), hipMemcpyDeviceToDevice, stream(s)));
+        HC_CHECK(hipMemcpyAsync(counts, splitCounts_.ptr, size_t(L) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                stream(s)));
+        // leaf keys are distinct, so any sort orders them the same way; the buffers of the splits serve as scratch
+        if (auto e = splitSortWs_.reserve(A::sortByKeyWs(size_t(L)))) return e;
+        if (auto e = A::sortByKey(s, tree, counts, size_t(L), splitKeys_.as<K>(), splitCounts_.as<uint32_t>(),
+                                  splitSortWs_.ptr, splitSortWs_.bytes))
+            return e;
+        HC_CHECK(hipMemcpyAsync(tree + L, &rootKeys[1], sizeof(K), hipMemcpyHostToDevice, stream(s)));
+        HC_CHECK(hipStreamSynchronize(stream(s))); // rootKeys lives on this stack frame
+        numLeaves_      = L;
+        convergedCalls_ = levels;
+        return HIPCSTONE_OK;
+    }
+
+    // grow a device buffer to `bytes`, keeping its first `keep` bytes
+    static hipcstone_status growKeep(DeviceBuf& b, size_t bytes, size_t keep, hipcstone_stream s)
+    {
+        if (bytes <= b.bytes) return HIPCSTONE_OK;
+        DeviceBuf nb;
+        if (auto e = nb.reserve(std::max(bytes, 2 * b.bytes))) return e;
+        if (keep) HC_CHECK(hipMemcpyAsync(nb.ptr, b.ptr, keep, hipMemcpyDeviceToDevice, stream(s)));
+        HC_CHECK(hipStreamSynchronize(stream(s))); // the old buffer is freed below
+        swap(b, nb);
         return HIPCSTONE_OK;
     }
 
@@ -424,6 +573,7 @@ private:
     DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_;
     DeviceBuf xyzh_[2][4];
     DeviceBuf tree_[2], counts_[2], updWs_;
+    DeviceBuf frontier_[2], splitKeys_, splitCounts_, splitCounters_, splitSortWs_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_;
     DeviceBuf assignOff_, assignBnd_, assignCnt_, assignWs_, flags_, searchC_, searchS_, localLayout_, haloWs_;

@@ -456,7 +456,7 @@ typedef struct
     int32_t num_leaves;
     int32_t num_nodes;
     int32_t num_internal;
-    int32_t converged_calls; /* updateOctree calls of the last full build */
+    int32_t converged_calls; /* tree levels the last full build split (it builds the converged tree directly) */
     void* keys;              /* sorted keys (K) */
     void* perm;              /* sorted-to-input permutation (u32) */
     void* x;                 /* sorted coordinates (T) */
