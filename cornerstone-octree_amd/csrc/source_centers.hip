@@ -41,20 +41,54 @@ __device__ inline Center4<T> normalizeMass(Center4<T> c)
     return c;
 }
 
+// one block per 256 consecutive leaves; their particles are one contiguous range. If it fits, the block stages it
+// in LDS with coalesced loads first (each thread's own leaf is then summed from LDS, still in index order); a block
+// whose range is larger reads its particles from global memory directly
+constexpr int kMcLeaves     = 256;
+constexpr int kMcStageBytes = 32768; // LDS staging per block: 2048 fp32 or 1024 fp64 particles
+
 template<class T>
-__global__ __launch_bounds__(256) void leafMassCentersKernel(const T* __restrict__ x, const T* __restrict__ y,
-                                                             const T* __restrict__ z, const T* __restrict__ m,
-                                                             const int32_t* __restrict__ leafToInternal,
-                                                             int32_t numLeaves, const uint32_t* __restrict__ layout,
-                                                             Center4<T>* __restrict__ centers)
+__global__ __launch_bounds__(kMcLeaves) void leafMassCentersKernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                                   const T* __restrict__ z, const T* __restrict__ m,
+                                                                   const int32_t* __restrict__ leafToInternal,
+                                                                   int32_t numLeaves,
+                                                                   const uint32_t* __restrict__ layout,
+                                                                   Center4<T>* __restrict__ centers)
 {
-    int32_t leaf = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    constexpr uint32_t kStage = kMcStageBytes / sizeof(Center4<T>);
+    __shared__ Center4<T> stage[kStage];
+    const int32_t first = int32_t(blockIdx.x) * kMcLeaves;
+    const int32_t last  = first + kMcLeaves < numLeaves ? first + kMcLeaves : numLeaves;
+    const int32_t leaf  = first + int32_t(threadIdx.x);
+    const uint32_t base = layout[first];
+    const uint32_t span = layout[last] - base;
+    const bool staged   = span <= kStage;
+    if (staged)
+    {
+        for (uint32_t i = threadIdx.x; i < span; i += kMcLeaves)
+        {
+            const uint32_t g = base + i;
+            stage[i]         = Center4<T>{x[g], y[g], z[g], m[g]};
+        }
+    }
+    __syncthreads();
     if (leaf >= numLeaves) return;
     Center4<T> c{0, 0, 0, 0};
-    const uint32_t e = layout[leaf + 1];
-    for (uint32_t i = layout[leaf]; i < e; ++i)
+    const uint32_t b = layout[leaf], e = layout[leaf + 1];
+    if (staged)
     {
-        addBody(c, x[i], y[i], z[i], m[i]);
+        for (uint32_t i = b - base; i < e - base; ++i)
+        {
+            const Center4<T> p = stage[i];
+            addBody(c, p.x, p.y, p.z, p.m);
+        }
+    }
+    else
+    {
+        for (uint32_t i = b; i < e; ++i)
+        {
+            addBody(c, x[i], y[i], z[i], m[i]);
+        }
     }
     centers[leafToInternal[leaf]] = normalizeMass(c);
 }
@@ -153,7 +187,7 @@ static hipcstone_status leafMassCenters(hipcstone_stream s, const T* x, const T*
     if (numLeaves <= 0) return HIPCSTONE_OK;
     if (!x || !y || !z || !m || !leafToInternal || !layout || !centers)
         return invalid("leafMassCenters: null pointer");
-    leafMassCentersKernel<T><<<ceilDiv(size_t(numLeaves), 256), 256, 0, stream(s)>>>(
+    leafMassCentersKernel<T><<<ceilDiv(size_t(numLeaves), kMcLeaves), kMcLeaves, 0, stream(s)>>>(
         x, y, z, m, leafToInternal, numLeaves, layout, reinterpret_cast<Center4<T>*>(centers));
     HC_CHECK_LAUNCH("leafMassCentersKernel");
     return HIPCSTONE_OK;
