@@ -281,20 +281,27 @@ struct BoxOf<double>
 };
 
 // sfc/sfc.hpp:142-179 sfc3D: ix = int(floor(x*mx) - xmin*mx) with mx = 2^L * ilx, clamped to 2^L - 1
-template<class K, int Curve, class T, class Box>
-__host__ __device__ inline K sfc3D(T x, T y, T z, const Box& box)
+template<class K, class T, class Box>
+__host__ __device__ inline void sfcGridCoords(T x, T y, T z, const Box& box, int& ix, int& iy, int& iz)
 {
     constexpr unsigned cubeLength = 1u << maxLevel<K>();
     constexpr int mcoord          = int(cubeLength - 1);
     T mx                          = T(cubeLength) * box.ilen[0];
     T my                          = T(cubeLength) * box.ilen[1];
     T mz                          = T(cubeLength) * box.ilen[2];
-    int ix                        = int(floor(x * mx) - box.lim[0] * mx);
-    int iy                        = int(floor(y * my) - box.lim[2] * my);
-    int iz                        = int(floor(z * mz) - box.lim[4] * mz);
+    ix                            = int(floor(x * mx) - box.lim[0] * mx);
+    iy                            = int(floor(y * my) - box.lim[2] * my);
+    iz                            = int(floor(z * mz) - box.lim[4] * mz);
     ix                            = ix < mcoord ? ix : mcoord;
     iy                            = iy < mcoord ? iy : mcoord;
     iz                            = iz < mcoord ? iz : mcoord;
+}
+
+template<class K, int Curve, class T, class Box>
+__host__ __device__ inline K sfc3D(T x, T y, T z, const Box& box)
+{
+    int ix, iy, iz;
+    sfcGridCoords<K>(x, y, z, box, ix, iy, iz);
     if constexpr (Curve == HIPCSTONE_HILBERT) { return iHilbert<K>(unsigned(ix), unsigned(iy), unsigned(iz)); }
     else { return iMorton<K>(unsigned(ix), unsigned(iy), unsigned(iz)); }
 }

@@ -45,6 +45,30 @@ def test_keys(hip, oracle, kd, curve, T):
         assert np.array_equal(dk.download(), oracle.compute_sfc_keys(x, y, z, obox, kd, curve))
 
 
+@pytest.mark.parametrize("kd", KDS)
+@pytest.mark.parametrize("T", TS)
+def test_keys_uniform_large(hip, oracle, kd, T):
+    """Every grid cell pattern of the Hilbert state machine: uniform points over the whole box at 4.5M particles
+    (several passes of the 4096-block grid-stride launch), with the box corners, faces and points past the
+    upper bound included, and every 5th key a removal marker that must be kept."""
+    n = 4_500_007
+    rng = np.random.default_rng(1234)
+    pts = rng.uniform(-0.5, 1.5, size=(3, n)).astype(T)
+    pts[:, :8] = np.array([[-0.5, 1.5][(c >> s) & 1] for c in range(8) for s in (2, 1, 0)], dtype=T).reshape(8, 3).T
+    pts[:, 8:16] = np.nextafter(T(1.5), T(0))
+    pts[0, 16:32] = T(1.7)
+    x, y, z = (np.ascontiguousarray(p) for p in pts)
+    gbox, obox = boxes(hip, T, -0.5, 1.5)
+    rm = kd(1) << kd(3 * (10 if kd == np.uint32 else 21))
+    keys_in = np.zeros(n, dtype=kd)
+    keys_in[::5] = rm
+    for curve in (HILBERT, MORTON):
+        dk = dev(hip, keys_in)
+        hip.compute_sfc_keys(dev(hip, x), dev(hip, y), dev(hip, z), dk, gbox, curve, honor_remove_key=True)
+        expect = oracle.compute_sfc_keys(x, y, z, obox, kd, curve, keys_in=keys_in)
+        assert np.array_equal(dk.download(), expect)
+
+
 def test_keys_remove_key(hip, oracle):
     T, kd = np.float64, np.uint64
     x = np.linspace(0.01, 0.99, 1000)
