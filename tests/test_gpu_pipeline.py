@@ -85,3 +85,56 @@ def test_pipeline_reuses_its_own_sorted_arrays(hip, oracle):
     assert np.array_equal(second["perm"], np.arange(n, dtype=np.uint32))
     assert np.array_equal(second["x"], first["x"])
     assert np.array_equal(second["leaves"], first["leaves"])
+
+
+@pytest.mark.parametrize("kd,T,curve", [(np.uint64, np.float32, HILBERT), (np.uint32, np.float64, MORTON)])
+def test_pipeline_warm_sort_far_from_previous_order(hip, oracle, kd, T, curve):
+    """Warm steps sort through the buckets of the previous leaf array; the result must be the stable sort's for
+    inputs far from the previous order: a shuffled input, coincident particles (equal keys, kept in input order),
+    thousands of particles crowded into one previous leaf (buckets of several chunks), a changed particle count and
+    a small displacement."""
+    from hipcstone import inputs
+
+    n = 60000
+    x, y, z, r = inputs.plummer(n, 0.05, seed=91, dtype=T)
+    h = inputs.plummer_h(r, n, 0.05, 60, dtype=T)
+    gbox, obox = hip.make_box(-1.0, 1.0, T), orc_box(-1.0, 1.0, T)
+    bucket = 16
+    pipe = hip.Pipeline(curve=curve, key_bits=64 if kd == np.uint64 else 32,
+                        real_bits=32 if T == np.float32 else 64, bucket_size=bucket, ngmax=0)
+    rng = np.random.default_rng(8)
+    tree = counts = None
+    for step in range(6):
+        d = [hip.DeviceArray.from_numpy(np.ascontiguousarray(a)) for a in (x, y, z, h)]
+        pipe.sync(*d, x.size, gbox, full_build=(step == 0))
+        keys = oracle.compute_sfc_keys(x, y, z, obox, kd, curve)
+        ks, perm = oracle.sort_by_key(keys)
+        if tree is None:
+            tree, counts = oracle.compute_octree(ks, bucket)
+        else:
+            tree, counts, _ = oracle.update_octree(ks, bucket, tree, counts)
+        got = pipe.arrays()
+        assert np.array_equal(got["keys"], ks), step
+        assert np.array_equal(got["perm"], perm), step
+        for k, a in (("x", x), ("y", y), ("z", z), ("h", h)):
+            assert np.array_equal(got[k], a[perm]), (step, k)
+        assert np.array_equal(got["leaves"], tree) and np.array_equal(got["counts"], counts), step
+        x, y, z, h = (a[perm].copy() for a in (x, y, z, h))
+        m = x.size
+        if step == 0:  # shuffled
+            p = rng.permutation(m)
+            x, y, z, h = x[p], y[p], z[p], h[p]
+        elif step == 1:  # every third particle onto its predecessor
+            for a in (x, y, z):
+                a[1::3] = a[0:m - 1:3][: a[1::3].size]
+        elif step == 2:  # 5000 particles into one tiny cube
+            sel = rng.choice(m, 5000, replace=False)
+            for a in (x, y, z):
+                a[sel] = (T(0.3) + rng.random(5000).astype(T) * T(1e-6)).astype(T)
+        elif step == 3:  # fewer particles
+            x, y, z, h = x[:-777], y[:-777], z[:-777], h[:-777]
+        else:  # small displacement
+            disp = (rng.random((3, m)) * 2 - 1).astype(T) * (T(1e-3) * h)
+            x = np.clip(x + disp[0], T(-1), T(1)).astype(T)
+            y = np.clip(y + disp[1], T(-1), T(1)).astype(T)
+            z = np.clip(z + disp[2], T(-1), T(1)).astype(T)
