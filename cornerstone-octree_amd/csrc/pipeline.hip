@@ -416,25 +416,32 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
         zo[pos]      = z[id];
         ho[pos]      = h[id];
     };
+    // the window at p: each lane's element, its bucket [r.x, r.y) and input index
+    uint2 r;
+    K k;
+    uint32_t id;
+    auto load = [&](uint32_t p) {
+        const uint32_t e = p + uint32_t(lane);
+        const bool valid = e < end;
+        r                = valid ? rangeIn[e] : make_uint2(0u, 0u);
+        k                = valid ? keysIn[e] : K(0);
+        id               = valid ? idxIn[e] : 0u;
+    };
     uint32_t P = uniform(segOffset[b0]);
+    if (P < end) load(P);
     while (P < end)
     {
-        const uint32_t e  = P + uint32_t(lane);
-        const bool valid  = e < end;
-        const uint2 r     = valid ? rangeIn[e] : make_uint2(0u, 0u); // the element's bucket [s0, s1)
-        const uint32_t s0 = r.x;
-        const uint32_t s1 = r.y;
-        const uint32_t firstEnd = uniform(s1); // lane 0 is valid and starts a bucket: s0 == P
+        const uint32_t firstEnd = uniform(r.y); // lane 0 holds element P, the first of its bucket
         if (firstEnd - P > uint32_t(kWave))
         {
             // a bucket of more than 64 elements, [P, firstEnd)
             const uint32_t cnt = firstEnd - P;
             for (uint32_t c0 = 0; c0 < cnt; c0 += kWave)
             {
-                const bool mine   = c0 + uint32_t(lane) < cnt;
-                const K k         = mine ? keysIn[P + c0 + lane] : K(0);
-                const uint32_t id = mine ? idxIn[P + c0 + lane] : 0u;
-                uint32_t rank     = 0;
+                const bool mine    = c0 + uint32_t(lane) < cnt;
+                const K kc         = mine ? keysIn[P + c0 + lane] : K(0);
+                const uint32_t idc = mine ? idxIn[P + c0 + lane] : 0u;
+                uint32_t rank      = 0;
                 for (uint32_t c1 = 0; c1 < cnt; c1 += kWave)
                 {
                     const uint32_t m = c1 + uint32_t(lane);
@@ -450,36 +457,55 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
                     {
                         const K kq        = wKey[q];
                         const uint32_t iq = wIdx[q];
-                        rank += uint32_t(kq < k || (kq == k && iq < id));
+                        rank += uint32_t(kq < kc || (kq == kc && iq < idc));
                     }
                 }
-                if (mine) put(P + rank, k, id);
+                if (mine) put(P + rank, kc, idc);
             }
             P = firstEnd;
+            if (P < end) load(P);
             continue;
         }
-        // whole buckets of the window: a prefix of its lanes (buckets are contiguous)
-        const bool whole  = valid && s1 <= P + uint32_t(kWave);
-        const K k         = valid ? keysIn[e] : K(0);
-        const uint32_t id = valid ? idxIn[e] : 0u;
+        // whole buckets of the window: a prefix of its lanes (buckets are contiguous); the next window starts after
+        // the last of them
+        const bool whole    = r.y != 0u && r.y <= P + uint32_t(kWave);
+        const int lastWhole = __popcll(ballot(whole)) - 1; // >= 0: lane 0's bucket fits
+        const uint32_t Pn   = uint32_t(__builtin_amdgcn_readlane(int(r.y), lastWhole));
+        // this window's x,y,z,h and the next window's elements are loaded while the ranks are counted
+        T xv = T(0), yv = T(0), zv = T(0), hv = T(0);
+        if (whole)
+        {
+            xv = x[id];
+            yv = y[id];
+            zv = z[id];
+            hv = h[id];
+        }
+        const uint2 rc     = r;
+        const K kc         = k;
+        const uint32_t idc = id;
+        if (Pn < end) load(Pn);
         waveSync();
-        wKey[lane] = k;
-        wIdx[lane] = id;
+        wKey[lane] = kc;
+        wIdx[lane] = idc;
         waveSync();
         if (whole)
         {
             uint32_t rank = 0;
-            for (uint32_t q = s0 - P; q < s1 - P; ++q)
+            for (uint32_t q = rc.x - P; q < rc.y - P; ++q)
             {
                 const K kq        = wKey[q];
                 const uint32_t iq = wIdx[q];
-                rank += uint32_t(kq < k || (kq == k && iq < id));
+                rank += uint32_t(kq < kc || (kq == kc && iq < idc));
             }
-            put(s0 + rank, k, id);
+            const uint32_t pos = rc.x + rank;
+            keysOut[pos]       = kc;
+            permOut[pos]       = idc;
+            xo[pos]            = xv;
+            yo[pos]            = yv;
+            zo[pos]            = zv;
+            ho[pos]            = hv;
         }
-        // the next window starts after the last whole bucket
-        const int lastWhole = __popcll(ballot(whole)) - 1; // >= 0: lane 0's bucket fits
-        P                   = uint32_t(__builtin_amdgcn_readlane(int(s1), lastWhole));
+        P = Pn;
     }
 }
 
