@@ -19,6 +19,7 @@
 #include "host_util.hpp"
 #include "runtime.hpp"
 #include "scan.hpp"
+#include "bucket_sort.hpp"
 
 namespace hipcstone
 {
@@ -514,9 +515,20 @@ public:
     } while (0)
         HC_STAGE(0, updateBox(s, x, y, z));
         HC_STAGE(1, A::keys(s, x, y, z, keys_.as<K>(), n, &box_));
-        HC_STAGE(2, A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xyzh_[0].as<T>(),
-                                  xyzh_[1].as<T>(), xyzh_[2].as<T>(), xyzh_[3].as<T>(), keyBuf_.as<K>(),
-                                  valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes));
+        if (numGlobal_ > 0)
+        {
+            // later calls: through the buckets of the previous global leaf array (same result as the radix sort)
+            HC_STAGE(2, (sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, gTree_[gCur_].template as<K>(), numGlobal_, x, y,
+                                                 z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(), xyzh_[2].as<T>(),
+                                                 xyzh_[3].as<T>(), perm_.as<uint32_t>(), keyBuf_.as<K>(),
+                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_)));
+        }
+        else
+        {
+            HC_STAGE(2, A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xyzh_[0].as<T>(),
+                                      xyzh_[1].as<T>(), xyzh_[2].as<T>(), xyzh_[3].as<T>(), keyBuf_.as<K>(),
+                                      valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes));
+        }
         HC_STAGE(3, globalUpdate(s));
         if (first_)
         {
@@ -989,7 +1001,7 @@ private:
     int32_t fL_ = 0, fN_ = 0;
     int globalSteps_ = 0, focusSteps_ = 0, focusConverged_ = 0;
 
-    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, mmPartials_, mmOut_, small_;
+    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, mmPartials_, mmOut_, small_, bsWs_, bsScanWs_;
     DeviceBuf xyzh_[4];
     DeviceBuf gTree_[2], gCounts_[2], updWs_;
     DeviceBuf fLeaves_, fLeavesNew_, fLeafCounts_, fNodeCounts_, leafOps_, ops_, prot_;

@@ -21,6 +21,7 @@
 #include "runtime.hpp"
 #include "host_util.hpp"
 #include "scan.hpp"
+#include "bucket_sort.hpp"
 
 
 namespace hipcstone
@@ -201,314 +202,6 @@ __global__ void setU32PairKernel(uint32_t* p, uint32_t a, uint32_t b)
     p[1] = b;
 }
 
-// ---- warm-step sort by the previous leaf array.
-// The stable sort of keys with values iota (gather.hpp:42-67, primitives_gpu.cu:310-367) is the unique ordering by
-// (key, input index). Any partition of the key space into ordered ranges gives it as the concatenation of each
-// range's elements ordered by (key, index); the previous step's leaf array is such a partition, and after a small
-// displacement each of its leaves holds about bucketSize keys. So a warm step buckets every key into the previous
-// leaf that contains it (counted with atomics, in any order), scatters the buckets contiguously, and orders each bucket
-// by (key, index) in one wave, gathering x,y,z,h on the way. The result equals the radix sort's for any input: the
-// previous tree only decides how the work is split.
-
-template<class K>
-__device__ inline K waveMinKey(K v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1)
-    {
-        K o;
-        if constexpr (sizeof(K) == 8)
-        {
-            uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), off, kWave));
-            uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), off, kWave));
-            o           = K((uint64_t(hi) << 32) | lo);
-        }
-        else { o = K(uint32_t(__shfl_xor(int(v), off, kWave))); }
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-template<class K>
-__device__ inline K waveMaxKey(K v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1)
-    {
-        K o;
-        if constexpr (sizeof(K) == 8)
-        {
-            uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), off, kWave));
-            uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), off, kWave));
-            o           = K((uint64_t(hi) << 32) | lo);
-        }
-        else { o = K(uint32_t(__shfl_xor(int(v), off, kWave))); }
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
-// one round of a 64-ary search for the wave-uniform key k with leaves[lo] <= k < leaves[hi]: pivot of this lane
-// (span <= 64: lo + lane, exact in one round; else 64 strictly increasing pivots with pivot 0 = lo)
-__device__ inline uint32_t searchPivot(uint32_t lo, uint32_t hi, int lane, bool& on)
-{
-    const uint32_t span = hi - lo;
-    on                  = span > 1 && (span <= uint32_t(kWave) ? uint32_t(lane) < span : true);
-    const uint32_t piv  = span <= uint32_t(kWave) ? lo + uint32_t(lane) : lo + uint32_t((uint64_t(span) * uint32_t(lane)) >> 6);
-    return on ? piv : lo;
-}
-
-// update of the bracket from the ballot of (pivot <= k): lanes 0..c-1, c >= 1 (pivot 0 is lo)
-__device__ inline void searchNarrow(uint32_t& lo, uint32_t& hi, uint32_t piv, uint64_t le)
-{
-    if (hi - lo <= 1) return;
-    const uint32_t span = hi - lo;
-    const int c         = __popcll(le);
-    const uint32_t nlo  = uint32_t(__builtin_amdgcn_readlane(int(piv), c - 1));
-    if (span <= uint32_t(kWave)) hi = nlo + 1;
-    else if (c < kWave) hi = uint32_t(__builtin_amdgcn_readlane(int(piv), c));
-    lo = nlo;
-}
-
-// the leaves holding the wave-uniform keys kmin <= kmax: both 64-ary searches advance in the same rounds, so their
-// pivot loads overlap
-template<class K>
-__device__ inline void waveLeafRange(const K* __restrict__ leaves, uint32_t L, K kmin, K kmax, int lane, uint32_t& a,
-                                     uint32_t& b)
-{
-    uint32_t lo0 = 0, hi0 = L, lo1 = 0, hi1 = L;
-    while (hi0 - lo0 > 1 || hi1 - lo1 > 1)
-    {
-        bool on0, on1;
-        const uint32_t p0 = searchPivot(lo0, hi0, lane, on0);
-        const uint32_t p1 = searchPivot(lo1, hi1, lane, on1);
-        const K v0        = leaves[p0];
-        const K v1        = leaves[p1];
-        const uint64_t l0 = ballot(on0 && v0 <= kmin);
-        const uint64_t l1 = ballot(on1 && v1 <= kmax);
-        searchNarrow(lo0, hi0, p0, l0);
-        searchNarrow(lo1, hi1, p1, l1);
-    }
-    a = lo0;
-    b = lo1;
-}
-
-// bucket[i] = the leaf of the previous leaf array holding keys[i]; slot[i] = its position among its bucket's keys
-// (any order). segCount[leaf] counts them; it starts at zero.
-template<class K>
-__global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ keys, uint32_t n,
-                                                          const K* __restrict__ leaves, int32_t L,
-                                                          uint32_t* __restrict__ segCount, uint32_t* __restrict__ bucket,
-                                                          uint32_t* __restrict__ slot)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    const int lane   = laneId();
-    const bool valid = i < n;
-    const K key      = valid ? keys[i] : K(0);
-    if (ballot(valid) == 0) return; // wave-uniform
-    // the leaf range holding the wave's keys, then a binary search inside it per lane
-    // (keys lie in [leaves[0], leaves[L]) = [0, nodeRange(0)))
-    const K kmin    = waveMinKey(valid ? key : ~K(0));
-    const K kmax    = waveMaxKey(valid ? key : K(0));
-    uint32_t a, b;
-    waveLeafRange(leaves, uint32_t(L), kmin, kmax, lane, a, b);
-    uint32_t lo = a; // leaves[a] <= key < leaves[b + 1]
-    if (b - a <= 16)
-    {
-        // few leaves: count the boundaries at or below the key (wave-uniform addresses)
-        for (uint32_t j = a + 1; j <= b; ++j)
-            lo += uint32_t(leaves[j] <= key);
-    }
-    else
-    {
-        uint32_t hi = b + 1;
-        while (hi - lo > 1)
-        {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (leaves[mid] <= key) lo = mid;
-            else hi = mid;
-        }
-    }
-    // one atomic per distinct leaf of the wave: the first lane of each group adds the group's size
-    uint64_t todo   = ballot(valid);
-    uint32_t myBase = 0; // the atomic's result, in the leader lane of each group
-    uint32_t myRank = 0; // rank among the group's lanes
-    int myLeader    = 0;
-    while (todo)
-    {
-        const int leader     = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t g     = uint32_t(__builtin_amdgcn_readlane(int(lo), leader));
-        const uint64_t peers = ballot(valid && lo == g);
-        if (lane == leader) myBase = atomicAdd(&segCount[g], uint32_t(__popcll(peers)));
-        if (laneIn(peers))
-        {
-            myLeader = leader;
-            myRank   = laneRank(peers);
-        }
-        todo &= ~peers;
-    }
-    const uint32_t base = uint32_t(__shfl(int(myBase), myLeader, kWave));
-    if (valid)
-    {
-        bucket[i] = lo;
-        slot[i]   = base + myRank;
-    }
-}
-
-__global__ void zeroU32Kernel(uint32_t* p, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
-template<class K>
-__global__ __launch_bounds__(256) void scatterByBucketKernel(const K* __restrict__ keys, uint32_t n,
-                                                             const uint32_t* __restrict__ bucket,
-                                                             const uint32_t* __restrict__ slot,
-                                                             const uint32_t* __restrict__ segOffset,
-                                                             K* __restrict__ keysOut, uint32_t* __restrict__ idxOut,
-                                                             uint2* __restrict__ rangeOut)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t b   = bucket[i];
-    const uint32_t s0  = segOffset[b];
-    const uint32_t pos = s0 + slot[i];
-    keysOut[pos]       = keys[i];
-    idxOut[pos]        = i;
-    rangeOut[pos]      = make_uint2(s0, segOffset[b + 1]);
-}
-
-// Each wave orders the buckets [g * kBucketsPerWave, (g + 1) * kBucketsPerWave), which are contiguous in the scattered
-// arrays, 64 elements at a time: a window starting at a bucket boundary holds some whole buckets (and the start of
-// the next one, left for the next window); every lane of a whole bucket counts the elements of its bucket that come
-// before it in (key, index) order (the pairs are distinct) and writes its element, with x,y,z,h, to bucket start +
-// rank. A bucket of more than 64 elements is ordered by itself, its elements compared in chunks of 64.
-constexpr uint32_t kBucketsPerWave = 16;
-
-template<class K, class T>
-__global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restrict__ keysIn,
-                                                               const uint32_t* __restrict__ idxIn,
-                                                               const uint2* __restrict__ rangeIn,
-                                                               const uint32_t* __restrict__ segOffset, int32_t L,
-                                                               K* __restrict__ keysOut, uint32_t* __restrict__ permOut,
-                                                               const T* __restrict__ x, const T* __restrict__ y,
-                                                               const T* __restrict__ z, const T* __restrict__ h,
-                                                               T* __restrict__ xo, T* __restrict__ yo,
-                                                               T* __restrict__ zo, T* __restrict__ ho)
-{
-    __shared__ K shKey[256];
-    __shared__ uint32_t shIdx[256];
-    const int lane = laneId();
-    const int w    = threadIdx.x / kWave;
-    K* wKey        = shKey + w * kWave;
-    uint32_t* wIdx = shIdx + w * kWave;
-    const uint64_t g = uint64_t(blockIdx.x) * 4 + uint64_t(w);
-    if (g * kBucketsPerWave >= uint64_t(L)) return; // wave-uniform
-    const uint32_t b0  = uint32_t(g * kBucketsPerWave);
-    const uint32_t b1  = uint32_t(std::min<uint64_t>(uint64_t(b0) + kBucketsPerWave, uint64_t(L)));
-    const uint32_t end = uniform(segOffset[b1]);
-    auto put = [&](uint32_t pos, K k, uint32_t id) {
-        keysOut[pos] = k;
-        permOut[pos] = id;
-        xo[pos]      = x[id];
-        yo[pos]      = y[id];
-        zo[pos]      = z[id];
-        ho[pos]      = h[id];
-    };
-    // the window at p: each lane's element, its bucket [r.x, r.y) and input index
-    uint2 r;
-    K k;
-    uint32_t id;
-    auto load = [&](uint32_t p) {
-        const uint32_t e = p + uint32_t(lane);
-        const bool valid = e < end;
-        r                = valid ? rangeIn[e] : make_uint2(0u, 0u);
-        k                = valid ? keysIn[e] : K(0);
-        id               = valid ? idxIn[e] : 0u;
-    };
-    uint32_t P = uniform(segOffset[b0]);
-    if (P < end) load(P);
-    while (P < end)
-    {
-        const uint32_t firstEnd = uniform(r.y); // lane 0 holds element P, the first of its bucket
-        if (firstEnd - P > uint32_t(kWave))
-        {
-            // a bucket of more than 64 elements, [P, firstEnd)
-            const uint32_t cnt = firstEnd - P;
-            for (uint32_t c0 = 0; c0 < cnt; c0 += kWave)
-            {
-                const bool mine    = c0 + uint32_t(lane) < cnt;
-                const K kc         = mine ? keysIn[P + c0 + lane] : K(0);
-                const uint32_t idc = mine ? idxIn[P + c0 + lane] : 0u;
-                uint32_t rank      = 0;
-                for (uint32_t c1 = 0; c1 < cnt; c1 += kWave)
-                {
-                    const uint32_t m = c1 + uint32_t(lane);
-                    waveSync(); // the previous chunk's reads are done
-                    if (m < cnt)
-                    {
-                        wKey[lane] = keysIn[P + m];
-                        wIdx[lane] = idxIn[P + m];
-                    }
-                    waveSync();
-                    const uint32_t lim = cnt - c1 < uint32_t(kWave) ? cnt - c1 : uint32_t(kWave);
-                    for (uint32_t q = 0; q < lim; ++q)
-                    {
-                        const K kq        = wKey[q];
-                        const uint32_t iq = wIdx[q];
-                        rank += uint32_t(kq < kc || (kq == kc && iq < idc));
-                    }
-                }
-                if (mine) put(P + rank, kc, idc);
-            }
-            P = firstEnd;
-            if (P < end) load(P);
-            continue;
-        }
-        // whole buckets of the window: a prefix of its lanes (buckets are contiguous); the next window starts after
-        // the last of them
-        const bool whole    = r.y != 0u && r.y <= P + uint32_t(kWave);
-        const int lastWhole = __popcll(ballot(whole)) - 1; // >= 0: lane 0's bucket fits
-        const uint32_t Pn   = uint32_t(__builtin_amdgcn_readlane(int(r.y), lastWhole));
-        // this window's x,y,z,h and the next window's elements are loaded while the ranks are counted
-        T xv = T(0), yv = T(0), zv = T(0), hv = T(0);
-        if (whole)
-        {
-            xv = x[id];
-            yv = y[id];
-            zv = z[id];
-            hv = h[id];
-        }
-        const uint2 rc     = r;
-        const K kc         = k;
-        const uint32_t idc = id;
-        if (Pn < end) load(Pn);
-        waveSync();
-        wKey[lane] = kc;
-        wIdx[lane] = idc;
-        waveSync();
-        if (whole)
-        {
-            uint32_t rank = 0;
-            for (uint32_t q = rc.x - P; q < rc.y - P; ++q)
-            {
-                const K kq        = wKey[q];
-                const uint32_t iq = wIdx[q];
-                rank += uint32_t(kq < kc || (kq == kc && iq < idc));
-            }
-            const uint32_t pos = rc.x + rank;
-            keysOut[pos]       = kc;
-            permOut[pos]       = idc;
-            xo[pos]            = xv;
-            yo[pos]            = yv;
-            zo[pos]            = zv;
-            ho[pos]            = hv;
-        }
-        P = Pn;
-    }
-}
-
 template<class K, class T>
 class Pipeline final : public PipelineBase
 {
@@ -685,40 +378,13 @@ public:
     }
 
 private:
-    // warm-step stable sort + gather through the buckets of the previous leaf array (numLeaves_ > 0); the keys of
-    // this step are in keys_ and lie in [leaves[0], leaves[L])
+    // warm-step stable sort + gather through the buckets of the previous leaf array (numLeaves_ > 0)
     hipcstone_status sortByPreviousLeaves(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, T* xo,
                                           T* yo, T* zo, T* ho, size_t n)
     {
-        const int32_t L = numLeaves_;
-        if (auto e = segWs_.reserve((4 * n + 2 * size_t(L) + 2) * sizeof(uint32_t))) return e;
-        if (auto e = scanWs_.reserve(hipcstone_scan_workspace_size(size_t(L)))) return e;
-        uint32_t* segCount  = segWs_.as<uint32_t>();
-        uint32_t* segOffset = segCount + L; // L + 1 entries
-        uint32_t* slot      = segOffset + L + 1;
-        uint32_t* idx       = slot + n;
-        // the ranges start at u32 offset 2L + 2 + 2n (one u32 of padding), 8-byte aligned
-        uint2* range        = reinterpret_cast<uint2*>(idx + n + 1);
-        uint32_t* bucket    = valBuf_.as<uint32_t>();
-        const K* leaves     = tree_[treeCur_].template as<K>();
-        K* keys             = keys_.as<K>();
-        K* bucketKeys       = keyBuf_.as<K>();
-        hipStream_t st      = stream(s);
-        const uint32_t n32  = uint32_t(n);
-
-        zeroU32Kernel<<<ceilDiv(size_t(L), 256), 256, 0, st>>>(segCount, uint32_t(L));
-        HC_CHECK_LAUNCH("zeroU32Kernel");
-        bucketByLeafKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, leaves, L, segCount, bucket, slot);
-        HC_CHECK_LAUNCH("bucketByLeafKernel");
-        if (auto e = hipcstone_leaf_layout(s, segCount, L, segOffset, scanWs_.ptr, scanWs_.bytes)) return e;
-        scatterByBucketKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, bucket, slot, segOffset, bucketKeys, idx,
-                                                                   range);
-        HC_CHECK_LAUNCH("scatterByBucketKernel");
-        const unsigned blocks = ceilDiv(ceilDiv(size_t(L), kBucketsPerWave), 4);
-        sortBucketsGatherKernel<K, T><<<blocks, 256, 0, st>>>(bucketKeys, idx, range, segOffset, L, keys,
-                                                              perm_.as<uint32_t>(), x, y, z, h, xo, yo, zo, ho);
-        HC_CHECK_LAUNCH("sortBucketsGatherKernel");
-        return HIPCSTONE_OK;
+        return sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, tree_[treeCur_].template as<K>(), numLeaves_, x, y, z, h,
+                                       xo, yo, zo, ho, perm_.as<uint32_t>(), keyBuf_.as<K>(), valBuf_.as<uint32_t>(),
+                                       segWs_, scanWs_);
     }
 
     hipcstone_status reserveParticles(size_t n)

@@ -65,6 +65,10 @@ def test_domain_sync_steps(hip, oracle, kd, T, bucket, bucket_focus, boundary):
         moved = [(a.download() + rng.normal(0, 0.005, n)).astype(T) for a in d[:3]]
         for a, m in zip(d[:3], moved):
             a.upload(np.clip(m, -1, 1).astype(T))
+        if step == 1:  # an input order far from the previous one (the later sorts bucket by the previous leaves)
+            p = rng.permutation(n)
+            for a in d:
+                a.upload(np.ascontiguousarray(a.download()[p]))
 
 
 def test_domain_neighbors_on_focus_tree(hip, oracle):
