@@ -165,6 +165,29 @@ __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ 
     }
 }
 
+// largest bucket: one atomicMax per block into *out (zeroed by the caller)
+template<int Unused = 0>
+__global__ __launch_bounds__(256) void maxBucketKernel(const uint32_t* __restrict__ counts, uint32_t n,
+                                                       uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t sh[4];
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+        m = counts[i] > m ? counts[i] : m;
+    for (int off = 32; off >= 1; off >>= 1)
+    {
+        const uint32_t o = uint32_t(__shfl_xor(int(m), off, kWave));
+        m                = o > m ? o : m;
+    }
+    if (laneId() == 0) sh[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const uint32_t b = std::max(std::max(sh[0], sh[1]), std::max(sh[2], sh[3]));
+        if (b > *out) atomicMax(out, b); // the pre-check skips most atomics
+    }
+}
+
 template<int Unused = 0>
 __global__ void zeroU32Kernel(uint32_t* p, uint32_t n)
 {
@@ -320,17 +343,28 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
     }
 }
 
+// ranking a bucket of s elements costs s * s / 64 compares in one wave; a larger bucket than this leaves the sort to
+// the radix sort
+constexpr uint32_t kMaxBucket = 2048;
+
 // keys[0..n) are sorted in place (stable, i.e. by (key, input index)), perm[i] is the input index of sorted element i
 // and xo..ho[i] = x..h[perm[i]]; leaves[0..L] is a cornerstone leaf array (leaves[0] = 0, L > 0). Keys at or above
-// leaves[L - 1] go to the last bucket. keyBuf and bucketBuf hold n elements; segWs and scanWs grow as needed.
+// leaves[L - 1] go to the last bucket. keyBuf and bucketBuf hold n elements; segWs and scanWs grow as needed. If some
+// bucket holds more than kMaxBucket keys, nothing is written but *done = false (one stream synchronization reads the
+// largest bucket).
 template<class K, class T>
 hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const K* leaves, int32_t L, const T* x,
                                    const T* y, const T* z, const T* h, T* xo, T* yo, T* zo, T* ho, uint32_t* perm,
-                                   K* keyBuf, uint32_t* bucketBuf, DeviceBuf& segWs, DeviceBuf& scanWs)
+                                   K* keyBuf, uint32_t* bucketBuf, DeviceBuf& segWs, DeviceBuf& scanWs, bool* done)
 {
-    if (n == 0) return HIPCSTONE_OK;
+    *done = false;
+    if (n == 0)
+    {
+        *done = true;
+        return HIPCSTONE_OK;
+    }
     if (L <= 0) return invalid("sortByLeafBuckets: empty leaf array");
-    if (auto e = segWs.reserve((4 * n + 2 * size_t(L) + 2) * sizeof(uint32_t))) return e;
+    if (auto e = segWs.reserve((4 * n + 2 * size_t(L) + 4) * sizeof(uint32_t))) return e;
     if (auto e = scanWs.reserve(hipcstone_scan_workspace_size(size_t(L)))) return e;
     uint32_t* segCount  = segWs.as<uint32_t>();
     uint32_t* segOffset = segCount + L; // L + 1 entries
@@ -341,10 +375,20 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
     hipStream_t st     = stream(s);
     const uint32_t n32 = uint32_t(n);
 
+    uint32_t* maxBucket = reinterpret_cast<uint32_t*>(range + n); // one u32 past the ranges
+
     zeroU32Kernel<<<ceilDiv(size_t(L), 256), 256, 0, st>>>(segCount, uint32_t(L));
+    HC_CHECK_LAUNCH("zeroU32Kernel");
+    zeroU32Kernel<<<1, 256, 0, st>>>(maxBucket, 1u);
     HC_CHECK_LAUNCH("zeroU32Kernel");
     bucketByLeafKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, leaves, L, segCount, bucketBuf, slot);
     HC_CHECK_LAUNCH("bucketByLeafKernel");
+    maxBucketKernel<<<std::min(ceilDiv(size_t(L), 256), 1024u), 256, 0, st>>>(segCount, uint32_t(L), maxBucket);
+    HC_CHECK_LAUNCH("maxBucketKernel");
+    uint32_t largest = 0;
+    HC_CHECK(hipMemcpyAsync(&largest, maxBucket, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HC_CHECK(hipStreamSynchronize(st));
+    if (largest > kMaxBucket) return HIPCSTONE_OK;
     if (auto e = hipcstone_leaf_layout(s, segCount, L, segOffset, scanWs.ptr, scanWs.bytes)) return e;
     scatterByBucketKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, bucketBuf, slot, segOffset, keyBuf, idx,
                                                                range);
@@ -353,6 +397,7 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
     sortBucketsGatherKernel<K, T><<<blocks, 256, 0, st>>>(keyBuf, idx, range, segOffset, L, keys, perm, x, y, z, h,
                                                           xo, yo, zo, ho);
     HC_CHECK_LAUNCH("sortBucketsGatherKernel");
+    *done = true;
     return HIPCSTONE_OK;
 }
 

@@ -515,15 +515,16 @@ public:
     } while (0)
         HC_STAGE(0, updateBox(s, x, y, z));
         HC_STAGE(1, A::keys(s, x, y, z, keys_.as<K>(), n, &box_));
+        bool bucketed = false;
         if (numGlobal_ > 0)
         {
             // later calls: through the buckets of the previous global leaf array (same result as the radix sort)
             HC_STAGE(2, (sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, gTree_[gCur_].template as<K>(), numGlobal_, x, y,
                                                  z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(), xyzh_[2].as<T>(),
                                                  xyzh_[3].as<T>(), perm_.as<uint32_t>(), keyBuf_.as<K>(),
-                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_)));
+                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_, &bucketed)));
         }
-        else
+        if (!bucketed)
         {
             HC_STAGE(2, A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xyzh_[0].as<T>(),
                                       xyzh_[1].as<T>(), xyzh_[2].as<T>(), xyzh_[3].as<T>(), keyBuf_.as<K>(),

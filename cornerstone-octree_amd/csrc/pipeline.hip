@@ -1,5 +1,5 @@
 // Host orchestration of the plain single-domain pipeline (C++ above the kernels):
-//   keys -> stable sort + gather x,y,z,h -> updateOctree (computeOctree on the first call) -> link ->
+//   keys -> stable sort + gather x,y,z,h (later calls: through the buckets of the previous leaf array) -> updateOctree (computeOctree on the first call) -> link ->
 //   geometric centers -> leaf layout -> findNeighbors,
 // and of one rank's share of the synced tree: makeSfcAssignment -> discoverHalos -> computeNodeLayout -> halo
 // gather (domain/domaindecomp.hpp:103-118, focus/octree_focus_mpi.hpp:511-582, halos/gather_halos_gpu.cu:28-57).
@@ -378,13 +378,19 @@ public:
     }
 
 private:
-    // warm-step stable sort + gather through the buckets of the previous leaf array (numLeaves_ > 0)
+    // warm-step stable sort + gather through the buckets of the previous leaf array (numLeaves_ > 0); the radix
+    // sort when a bucket is too large
     hipcstone_status sortByPreviousLeaves(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, T* xo,
                                           T* yo, T* zo, T* ho, size_t n)
     {
-        return sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, tree_[treeCur_].template as<K>(), numLeaves_, x, y, z, h,
-                                       xo, yo, zo, ho, perm_.as<uint32_t>(), keyBuf_.as<K>(), valBuf_.as<uint32_t>(),
-                                       segWs_, scanWs_);
+        bool done = false;
+        if (auto e = sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, tree_[treeCur_].template as<K>(), numLeaves_, x, y,
+                                             z, h, xo, yo, zo, ho, perm_.as<uint32_t>(), keyBuf_.as<K>(),
+                                             valBuf_.as<uint32_t>(), segWs_, scanWs_, &done))
+            return e;
+        if (done) return HIPCSTONE_OK;
+        return A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xo, yo, zo, ho, keyBuf_.as<K>(),
+                             valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes);
     }
 
     hipcstone_status reserveParticles(size_t n)

@@ -91,8 +91,8 @@ def test_pipeline_reuses_its_own_sorted_arrays(hip, oracle):
 def test_pipeline_warm_sort_far_from_previous_order(hip, oracle, kd, T, curve):
     """Warm steps sort through the buckets of the previous leaf array; the result must be the stable sort's for
     inputs far from the previous order: a shuffled input, coincident particles (equal keys, kept in input order),
-    thousands of particles crowded into one previous leaf (buckets of several chunks), a changed particle count and
-    a small displacement."""
+    1500 particles crowded into one previous leaf (a bucket of many chunks), a changed particle count, and 6000
+    crowded particles (a bucket above the limit: the radix sort takes over)."""
     from hipcstone import inputs
 
     n = 60000
@@ -127,14 +127,10 @@ def test_pipeline_warm_sort_far_from_previous_order(hip, oracle, kd, T, curve):
         elif step == 1:  # every third particle onto its predecessor
             for a in (x, y, z):
                 a[1::3] = a[0:m - 1:3][: a[1::3].size]
-        elif step == 2:  # 5000 particles into one tiny cube
-            sel = rng.choice(m, 5000, replace=False)
+        elif step in (2, 4):  # particles crowded into one tiny cube
+            c = 1500 if step == 2 else 6000
+            sel = rng.choice(m, c, replace=False)
             for a in (x, y, z):
-                a[sel] = (T(0.3) + rng.random(5000).astype(T) * T(1e-6)).astype(T)
+                a[sel] = (T(0.3 if step == 2 else -0.6) + rng.random(c).astype(T) * T(1e-6)).astype(T)
         elif step == 3:  # fewer particles
             x, y, z, h = x[:-777], y[:-777], z[:-777], h[:-777]
-        else:  # small displacement
-            disp = (rng.random((3, m)) * 2 - 1).astype(T) * (T(1e-3) * h)
-            x = np.clip(x + disp[0], T(-1), T(1)).astype(T)
-            y = np.clip(y + disp[1], T(-1), T(1)).astype(T)
-            z = np.clip(z + disp[2], T(-1), T(1)).astype(T)
