@@ -11,8 +11,8 @@
 // leaves and counts (csarray.hpp:408-426; performance/octree.cpp:43-44).
 //
 // All buffers are allocated at creation or when a capacity is exceeded (outside steady-state steps). A step
-// synchronizes the stream once per updateOctree call (the new leaf count decides launch sizes) and once to read
-// the level range.
+// synchronizes the stream once per updateOctree call (the new leaf count decides launch sizes), once to read
+// the level range and, on later calls, once to read the largest bucket of the sort.
 #include <algorithm>
 #include <memory>
 #include <vector>
