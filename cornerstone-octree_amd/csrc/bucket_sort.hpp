@@ -11,6 +11,33 @@
 namespace hipcstone
 {
 
+// rank += ((kq, iq) < (k, id)): the borrow out of the multi-word subtraction (kq:iq) - (k:id), added with carry
+__device__ inline uint32_t addIfPairLess(uint32_t rank, uint64_t kq, uint32_t iq, uint64_t k, uint32_t id)
+{
+    uint32_t t;
+    asm("v_sub_co_u32_e32 %[t], vcc, %[iq], %[id]\n\t"
+        "v_subb_co_u32_e32 %[t], vcc, %[ql], %[kl], vcc\n\t"
+        "v_subb_co_u32_e32 %[t], vcc, %[qh], %[kh], vcc\n\t"
+        "v_addc_co_u32_e32 %[r], vcc, 0, %[r], vcc"
+        : [t] "=&v"(t), [r] "+v"(rank)
+        : [iq] "v"(iq), [id] "v"(id), [ql] "v"(uint32_t(kq)), [kl] "v"(uint32_t(k)), [qh] "v"(uint32_t(kq >> 32)),
+          [kh] "v"(uint32_t(k >> 32))
+        : "vcc");
+    return rank;
+}
+
+__device__ inline uint32_t addIfPairLess(uint32_t rank, uint32_t kq, uint32_t iq, uint32_t k, uint32_t id)
+{
+    uint32_t t;
+    asm("v_sub_co_u32_e32 %[t], vcc, %[iq], %[id]\n\t"
+        "v_subb_co_u32_e32 %[t], vcc, %[kq], %[k], vcc\n\t"
+        "v_addc_co_u32_e32 %[r], vcc, 0, %[r], vcc"
+        : [t] "=&v"(t), [r] "+v"(rank)
+        : [iq] "v"(iq), [id] "v"(id), [kq] "v"(kq), [k] "v"(k)
+        : "vcc");
+    return rank;
+}
+
 // ---- warm-step sort by the previous leaf array.
 // The stable sort of keys with values iota (gather.hpp:42-67, primitives_gpu.cu:310-367) is the unique ordering by
 // (key, input index). Any partition of the key space into ordered ranges gives it as the concatenation of each
@@ -329,7 +356,7 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
             {
                 const K kq        = wKey[q];
                 const uint32_t iq = wIdx[q];
-                rank += uint32_t(kq < kc || (kq == kc && iq < idc));
+                rank              = addIfPairLess(rank, kq, iq, kc, idc);
             }
             const uint32_t pos = rc.x + rank;
             keysOut[pos]       = kc;
