@@ -631,3 +631,87 @@ class CpuDomain:
         c = self.lib.upsweep_centers(o["levelRange"], o["childOffsets"], c, o["levelRange"].size - 2)
         inv_theta = np.float32(1.0 / np.float64(np.float32(theta)))
         return self.lib.set_mac(o["prefixes"], c, float(inv_theta), self.box()), o
+
+
+REFDOM_PATH = os.path.join(HERE, "_ref", "libcstone_ref_domain.so")
+
+
+class RefDomain:
+    """The reference's own one-rank Domain<KeyType, T, Cpu> (domain/domain.hpp:38-420), run as an MPI singleton
+    through oracle/_ref/libcstone_ref_domain.so (oracle/ref_domain.cpp). sync() / sync_grav() return the same keys
+    as CpuDomain.sync, plus what only a run of the reference knows: its focus-tree layout, the linked focus tree with
+    its geometric centers, start/end indices and (syncGrav) the expansion centers. Only available where the
+    reference sources and MPI were present at build time."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(REFDOM_PATH)
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            lib = C.CDLL(REFDOM_PATH)
+            lib.refdom_create.restype = C.c_void_p
+            lib.refdom_last_error.restype = C.c_char_p
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, key_dtype, real_dtype, bucket, bucket_focus, lo, hi, boundary=(0, 0, 0), theta=0.5):
+        self.kd = np.dtype(key_dtype)
+        self.T = np.dtype(real_dtype).type
+        self.box0 = make_box(lo, hi, self.T, boundary)
+        lib = self.lib()
+        self.h = lib.refdom_create(C.c_int(8 * self.kd.itemsize), C.c_int(8 * np.dtype(self.T).itemsize),
+                                   C.c_uint(bucket), C.c_uint(bucket_focus), C.c_float(theta), C.byref(self.box0))
+        if not self.h:
+            raise RuntimeError("refdom_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib().refdom_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def _call(self, name, *args):
+        return getattr(self.lib(), "refdom_" + name)(C.c_void_p(self.h), *args)
+
+    def sync(self, x, y, z, h, m=None):
+        n = x.size
+        xs, ys, zs, hs = (np.array(a, dtype=self.T, copy=True) for a in (x, y, z, h))
+        ms = None if m is None else np.array(m, dtype=self.T, copy=True)
+        ids = np.zeros(n, np.uint32)
+        keys = np.zeros(n, self.kd)
+        rc = self._call("sync", _ptr(xs), _ptr(ys), _ptr(zs), _ptr(hs), _ptr(ms), _ptr(ids), C.c_size_t(n),
+                        _ptr(keys))
+        if rc != 0:
+            raise RuntimeError("reference Domain sync failed: " + self.lib().refdom_last_error().decode())
+        sz = np.zeros(7, np.int64)
+        self._call("sizes", _ptr(sz))
+        start, end, nwh, ng, nf, nn, ni = (int(v) for v in sz)
+        bx = np.zeros(6)
+        self._call("box", _ptr(bx))
+        gtree, gcounts = np.zeros(ng + 1, self.kd), np.zeros(ng, np.uint32)
+        self._call("global", _ptr(gtree), _ptr(gcounts))
+        fl, flc, fnc = np.zeros(nf + 1, self.kd), np.zeros(nf, np.uint32), np.zeros(nn, np.uint32)
+        layout = np.zeros(nf + 1, np.uint32)
+        self._call("focus", _ptr(fl), _ptr(flc), _ptr(fnc), _ptr(layout))
+        L = 11 if self.kd == np.uint32 else 22
+        pf, co, i2l = np.zeros(nn, self.kd), np.zeros(nn, np.int32), np.zeros(nn, np.int32)
+        lr = np.zeros(L + 1, np.int32)
+        cen, siz = np.zeros(3 * nn, self.T), np.zeros(3 * nn, self.T)
+        self._call("octree", _ptr(pf), _ptr(co), _ptr(i2l), _ptr(lr), _ptr(cen), _ptr(siz))
+        r = dict(keys=keys, perm=ids, x=xs, y=ys, z=zs, h=hs, lo=[self.T(bx[0]), self.T(bx[2]), self.T(bx[4])],
+                 hi=[self.T(bx[1]), self.T(bx[3]), self.T(bx[5])], gtree=gtree, gcounts=gcounts, fleaves=fl,
+                 fcounts=flc, fnodecounts=fnc, layout=layout, start=start, end=end, with_halos=nwh,
+                 prefixes=pf, childOffsets=co, internalToLeaf=i2l, levelRange=lr, centers=cen, sizes=siz,
+                 num_internal=ni)
+        if ms is not None:
+            ec = np.zeros(4 * nn, self.T)
+            self._call("expansion_centers", _ptr(ec))
+            r["m"] = ms
+            r["expansion_centers"] = ec
+        return r
+
+    def sync_grav(self, x, y, z, h, m):
+        return self.sync(x, y, z, h, m)

@@ -85,3 +85,44 @@ def neighbor_rows_equal(nb1, nc1, nb2, nc2, ngmax):
     cols = np.arange(ngmax)[None, :]
     mask = cols < m[:, None]
     return np.array_equal(np.where(mask, nb1, 0), np.where(mask, nb2, 0))
+
+
+# ---- the moving input of the one-rank Domain checks (reference Domain, CPU oracle, GPU, golden fixtures)
+
+# the one-rank Domain cases of tests/golden/domain.npz: (name, key dtype, real dtype, bucket, bucket focus,
+# boundary, theta, particles, calls, syncGrav, input seed)
+DOMAIN_GOLDEN_CASES = [
+    ("a", np.uint64, np.float32, 64, 16, (0, 0, 0), 0.5, 8000, 5, False, 201),
+    ("b", np.uint32, np.float64, 32, 8, (1, 2, 3), 0.5, 6000, 5, False, 202),
+    ("g", np.uint64, np.float64, 32, 8, (0, 0, 0), 0.7, 6000, 3, True, 203),
+]
+
+
+def domain_input(n, T, seed, with_mass=False):
+    """Plummer particles (a = 0.1) with h for about 60 neighbors, and optionally masses in [-0.4, 1.6) with 3 % zeros."""
+    from hipcstone import inputs
+
+    x, y, z, r = inputs.plummer(n, 0.1, seed=seed, dtype=T)
+    h = inputs.plummer_h(r, n, 0.1, 60, dtype=T)
+    if not with_mass:
+        return [x, y, z, h]
+    rng = np.random.default_rng(seed + 1)
+    m = (rng.random(n) * 2 - 0.4).astype(T)
+    m[rng.random(n) < 0.03] = 0
+    return [x, y, z, h, m]
+
+
+def domain_move(rng, out, step, carried=("h",), lo=-1.0, hi=1.0):
+    """The next step's input from a sync's output: x, y, z displaced, the `carried` arrays kept; a shuffled order
+    after step 1, a large displacement (a rearranged tree) after step 2, and after step 3 a contraction by 20 %,
+    which the box may follow by only 5 % per side and call (limitBoxShrinking, sfc/box.hpp:398-415)."""
+    T = out["x"].dtype.type
+    n = out["x"].size
+    sigma = 0.05 if step == 2 else 0.005
+    scale = 0.8 if step == 3 else 1.0
+    nxt = [np.clip(out[k] * T(scale) + rng.normal(0, sigma, n), lo, hi).astype(T) for k in "xyz"]
+    nxt += [np.array(out[k], copy=True) for k in carried]
+    if step == 1:
+        p = rng.permutation(n)
+        nxt = [a[p] for a in nxt]
+    return nxt

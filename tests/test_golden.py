@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 
 from conftest import load_golden
-from helpers import neighbor_rows_equal
-from oracle_py import HILBERT, MORTON
+from helpers import DOMAIN_GOLDEN_CASES, domain_input, domain_move, neighbor_rows_equal
+from oracle_py import HILBERT, MORTON, CpuDomain
 from oracle_py import make_box as orc_box
 
 VARIANTS = [(c, cs, kd, ks, T, ts) for c, cs in ((HILBERT, "hilbert"), (MORTON, "morton"))
@@ -59,6 +59,37 @@ def test_oracle_halos_golden(oracle):
     flags = oracle.find_halos(g["prefixes"], g["childOffsets"], g["parents"], g["centers"], g["sizes"], g["leaves"],
                               sc, ss, box, first, last)
     assert np.array_equal(flags, g["flags"])
+
+
+def _domain_check(g, name, step, got, T):
+    """One call's outputs against domain.npz: `got` holds keys, x/y/z/h (reordered), box, trees and layout."""
+    p = f"{name}{step}_"
+    assert np.array_equal(got["keys"], g[p + "keys"])
+    box = np.array([got["lo"][0], got["hi"][0], got["lo"][1], got["hi"][1], got["lo"][2], got["hi"][2]], dtype=T)
+    assert np.array_equal(box, g[p + "box"])
+    for k in ("gtree", "gcounts", "fleaves", "fcounts", "fnodecounts", "layout"):
+        assert np.array_equal(got[k], g[p + k]), (name, step, k)
+
+
+@pytest.mark.parametrize("case", DOMAIN_GOLDEN_CASES, ids=[c[0] for c in DOMAIN_GOLDEN_CASES])
+def test_oracle_domain_golden(oracle, case):
+    """The oracle's one-rank Domain sequence reproduces the reference Domain's calls stored in domain.npz."""
+    g = load_golden("domain.npz")
+    name, kd, T, bucket, bucket_focus, boundary, theta, n, calls, grav, seed = case
+    arrays = domain_input(n, T, seed, with_mass=grav)
+    assert np.array_equal([float(np.sum(a, dtype=np.float64)) for a in arrays], g[f"{name}_input_sum"]), \
+        "the input generator drifted from the one that wrote the fixture"
+    dom = CpuDomain(oracle, kd, T, bucket, bucket_focus, -1.0, 1.0, boundary)
+    rng = np.random.default_rng(seed + 1000)
+    for step in range(calls):
+        r = dom.sync_grav(*arrays, theta) if grav else dom.sync(*arrays)
+        assert np.array_equal(r["perm"], g[f"{name}{step}_perm"])
+        r["layout"] = np.concatenate([[0], np.cumsum(r["fcounts"])]).astype(np.uint32)
+        _domain_check(g, name, step, r, T)
+        if grav:
+            assert np.array_equal(r["expansion_centers"].view(np.uint8).ravel(),
+                                  g[f"{name}{step}_expansion_centers"].view(np.uint8).ravel())
+        arrays = domain_move(rng, r, step, carried=("h", "m") if grav else ("h",))
 
 
 # ---------------------------------------------------------------- GPU against the same fixtures
@@ -122,3 +153,41 @@ def test_gpu_halos_golden(hip):
                    _dev(hip, g["centers"].reshape(-1)), _dev(hip, g["sizes"].reshape(-1)), _dev(hip, g["leaves"]), sc,
                    ss, box, first, last, N, flags)
     assert np.array_equal(flags.download(), g["flags"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", DOMAIN_GOLDEN_CASES, ids=[c[0] for c in DOMAIN_GOLDEN_CASES])
+def test_gpu_domain_golden(hip, case):
+    """hipcstone_domain_sync / _sync_grav reproduce the reference Domain's calls stored in domain.npz, bit for bit:
+    box, keys, particle order, global and focus trees, layout and expansion centers."""
+    g = load_golden("domain.npz")
+    name, kd, T, bucket, bucket_focus, boundary, theta, n, calls, grav, seed = case
+    arrays = domain_input(n, T, seed, with_mass=grav)
+    assert np.array_equal([float(np.sum(a, dtype=np.float64)) for a in arrays], g[f"{name}_input_sum"]), \
+        "the input generator drifted from the one that wrote the fixture"
+    dom = hip.Domain(hip.make_box(-1.0, 1.0, T, boundary), key_bits=64 if kd == np.uint64 else 32,
+                     bucket_size=bucket, bucket_size_focus=bucket_focus, theta=theta)
+    rng = np.random.default_rng(seed + 1000)
+    keys_out = hip.DeviceArray(n, kd)
+    for step in range(calls):
+        print(f"domain golden case {name} call {step}", flush=True)
+        d = [_dev(hip, a) for a in arrays]
+        if grav:
+            dom.sync_grav(*d, keys_out=keys_out)
+        else:
+            dom.sync(*d, n, keys_out)
+        a = dom.arrays()
+        perm = g[f"{name}{step}_perm"]
+        got = dict(keys=keys_out.download(), lo=[T(v) for v in a["box_lo"]], hi=[T(v) for v in a["box_hi"]],
+                   gtree=a["global_leaves"], gcounts=a["global_counts"], fleaves=a["leaves"], fcounts=a["counts"],
+                   fnodecounts=a["node_counts"], layout=a["layout"])
+        _domain_check(g, name, step, got, T)
+        out = {}
+        for k, dv, hv in zip("xyzhm", d, arrays):
+            out[k] = dv.download()
+            assert np.array_equal(out[k], hv[perm]), (name, step, k)
+        if grav:
+            ec = dom.expansion_centers()
+            assert np.array_equal(ec.view(np.uint8).ravel(),
+                                  g[f"{name}{step}_expansion_centers"].view(np.uint8).ravel())
+        arrays = domain_move(rng, out, step, carried=("h", "m") if grav else ("h",))

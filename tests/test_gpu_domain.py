@@ -1,11 +1,11 @@
 """GPU parity of the one-rank Domain::sync (box, keys, order, global leaf array, focus tree, its octree, centers and
-layout) against the oracle's one-rank domain sequence over several steps, and the neighbor search on the focus
-tree."""
+layout) against the oracle's one-rank domain sequence (itself pinned to the reference's own Domain) over several
+steps, and the neighbor search on the focus tree."""
 
 import numpy as np
 import pytest
 
-from helpers import neighbor_rows_equal
+from helpers import domain_move, neighbor_rows_equal
 from oracle_py import CpuDomain
 
 pytestmark = pytest.mark.gpu
@@ -37,7 +37,7 @@ def test_domain_sync_steps(hip, oracle, kd, T, bucket, bucket_focus, boundary):
     rng = np.random.default_rng(32)
     d = [dev(hip, a) for a in (x, y, z, h)]
     keys_out = hip.DeviceArray(n, kd)
-    for step in range(4):
+    for step in range(6):
         print(f"domain step {step}: {np.dtype(kd).name} {np.dtype(T).name} bucket {bucket}/{bucket_focus} "
               f"boundary {boundary}", flush=True)
         xs, ys, zs, hs = (a.download() for a in d)
@@ -61,14 +61,10 @@ def test_domain_sync_steps(hip, oracle, kd, T, bucket, bucket_focus, boundary):
         assert np.array_equal(g["centers"], c) and np.array_equal(g["sizes"], s)
         assert np.array_equal(g["layout"], np.concatenate([[0], np.cumsum(o["fcounts"])]).astype(np.uint32))
         assert (v.start_index, v.end_index) == (0, n)
-        # move the particles for the next step (host-side, identical for both)
-        moved = [(a.download() + rng.normal(0, 0.005, n)).astype(T) for a in d[:3]]
-        for a, m in zip(d[:3], moved):
-            a.upload(np.clip(m, -1, 1).astype(T))
-        if step == 1:  # an input order far from the previous one (the later sorts bucket by the previous leaves)
-            p = rng.permutation(n)
-            for a in d:
-                a.upload(np.ascontiguousarray(a.download()[p]))
+        # the next input (host-side, identical for both): displaced; shuffled after step 1 (the later sorts bucket
+        # by the previous leaves), rearranged after step 2, contracted by 20 % after step 3 (the box shrink limit)
+        for a, m in zip(d, domain_move(rng, o, step)):
+            a.upload(np.ascontiguousarray(m))
 
 
 def test_domain_neighbors_on_focus_tree(hip, oracle):
