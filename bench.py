@@ -31,6 +31,52 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu), stated next to every CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def stage_report(stages, bytes_per_launch):
+    """Per-stage HIP-event times with each stage's algorithmic bytes per launch, achieved GB/s and HBM fraction."""
+    out = {}
+    for name, (ms, c) in stages.items():
+        if c <= 0:
+            continue
+        avg = ms / c
+        rec = {"avg_ms": round(avg, 4), "launches": c}
+        b = bytes_per_launch.get(name)
+        if b is not None and avg > 0:
+            gbs = b / (avg / 1e3) / 1e9
+            rec.update({"algorithmic_bytes": int(b), "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        out[name] = rec
+    return out
+
+
+def pipeline_stage_bytes(n, leaves, nodes, kbar, real_bytes=4, key_bytes=8):
+    """Algorithmic bytes per launch of each plain-pipeline stage: every input read once, every output written once.
+    keys: x,y,z in, key out; sort+gather: key read, key + u32 index written, index + x,y,z,h read and written;
+    tree update: ~32 B per leaf (leaves and counts in and out, one pass over the keys by searches); link: prefix in,
+    child offsets, internal/leaf maps and parents out per node; centers: prefix in, two Vec3 out per node; layout:
+    counts in, offsets out; neighbors: x,y,z,h in, count and the stored neighbor indices out."""
+    R, K = real_bytes, key_bytes
+    return {
+        "keys": n * (3 * R + K),
+        "sort_gather": n * (K + K + 4) + n * (4 + 4 * R + 4 * R),
+        "tree_update": 32.0 * leaves,
+        "link": nodes * (K + 4 + 4 + 4 + 0.5),
+        "centers": nodes * (K + 6 * R),
+        "layout": 8.0 * leaves,
+        "neighbors": n * (4 * R + 4) + 4.0 * kbar * n,
+    }
+
+
 def prepare_h(hip, pipe, x, y, z, h, n, box, target=100.0, iters=3):
     """Analytic initial h, then `iters` updates h <- h (target/n_i)^(1/3) from GPU neighbor counts.
     Returns host arrays in SFC order."""
@@ -74,6 +120,7 @@ def cpu_baseline(x, y, z, h, stride, seed):
         "value": x.size / (total / 1e3) / 1e6,
         "unit": "Mparticles/s",
         "cores": REF.omp_threads(),
+        "cpu_model": cpu_model(),
         "kind": "reference",
         "sample": (f"one warm step of the same {x.size}-particle workload through the reference CPU OpenMP path "
                    f"(oracle/_ref); keys/sort/tree/link/centers on all particles ({sum(t[:5]):.0f} ms), neighbors for "
@@ -81,6 +128,97 @@ def cpu_baseline(x, y, z, h, stride, seed):
         "stage_ms": {"keys": t[0], "sort_gather": t[1], "tree_update": t[2], "link": t[3], "centers_layout": t[4],
                      "neighbors_extrapolated": t[5] * stride},
     }
+
+
+def c1_bench(args):
+    """Plain CPU-runnable pipeline at its own size: 1,000,000 fp64 particles uniform in [0,1)^3, h for about 100
+    neighbors, Hilbert-64 keys, bucketSize 64; one step = a full build from the unsorted input (keys, stable sort +
+    gather, the converged leaf array from a single root, link, centers, layout) + findNeighbors with ngmax 150."""
+    import hipcstone as hip
+    from hipcstone import inputs
+
+    n, K, W, ngmax = 1000000, args.steps, args.warmup, 150
+    T = np.float64
+    x, y, z = inputs.uniform_cube(n, seed=42, dtype=T)
+    h = np.full(n, inputs.uniform_h(n, 100.0), dtype=T)
+    box = hip.make_box(0.0, 1.0, T)
+    pipe = hip.Pipeline(curve=hip.HILBERT, key_bits=64, real_bits=64, bucket_size=64, ngmax=ngmax)
+    d = [hip.DeviceArray.from_numpy(a) for a in (x, y, z, h)]
+    nb = hip.DeviceArray(n * ngmax, np.uint32)
+    nc = hip.DeviceArray(n, np.uint32)
+
+    def step():
+        pipe.sync(*d, n, box, full_build=True, neighbors=nb, counts=nc)
+
+    for _ in range(W):
+        step()
+    hip.device_sync()
+    events = [hip.Event() for _ in range(K + 1)]
+    pipe.set_timing(True)
+    hip.device_sync()
+    events[0].record()
+    for k in range(K):
+        step()
+        events[k + 1].record()
+    hip.device_sync()
+    per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
+    total_ms = events[0].elapsed_ms(events[K])
+    stages = pipe.stage_times()
+    pipe.set_timing(False)
+    counts = nc.download()
+    kbar = float(np.minimum(counts, ngmax).mean())
+    v = pipe.view()
+    nb_ms, nb_launch = stages["neighbors"]
+    nb_avg = nb_ms / max(nb_launch, 1)
+    sb = pipeline_stage_bytes(n, int(v.num_leaves), int(v.num_nodes), kbar, real_bytes=8)
+    achieved = sb["neighbors"] / (nb_avg / 1e3) / 1e9
+    step_bytes = float(sum(sb.values()))
+    ms_per_step = total_ms / K
+    result = {
+        "metric": "Mparticles/s keys→sort→tree→neighbors (1M fp64 uniform, full build)",
+        "value": round(n * K / (total_ms / 1e3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": ("plain CPU-runnable pipeline: 1,000,000 fp64 particles uniform in [0,1)^3, h = "
+                         "0.5 (100 / (4/3 pi N))^(1/3), Hilbert-64 keys, bucketSize 64, full build from a single "
+                         "root per step, findNeighbors ngmax 150"),
+            "particles": n, "leaves": int(v.num_leaves), "nodes": int(v.num_nodes),
+            "mean_neighbors": round(float(counts.mean()), 2), "max_neighbors": int(counts.max()),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
+                       "p99": round(float(np.percentile(per_step, 99)), 3)},
+        "roofline": {"kernel": "findNeighborsKernel", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(nb_avg, 3),
+                     "algorithmic_bytes_per_launch": sb["neighbors"]},
+        "pipeline_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                              "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
+                              "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "stages": stage_report(stages, sb),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_py import REF, make_box
+
+        if REF is not None:
+            stride = args.cpu_stride
+            t0 = time.time()
+            _, _, t, ncs = REF.pipeline(x.copy(), y.copy(), z.copy(), h.copy(), make_box(0.0, 1.0, T), 64, ngmax,
+                                        nb_stride=stride)
+            log(f"cpu baseline (c1): {time.time() - t0:.1f} s")
+            total = t[0] + t[1] + t[2] + t[3] + t[4] + t[5] * stride
+            result["cpu_baseline"] = {
+                "value": round(n / (total / 1e3) / 1e6, 4), "unit": "Mparticles/s", "cores": REF.omp_threads(),
+                "cpu_model": cpu_model(), "kind": "reference",
+                "sample": (f"one full build of the same 1M-particle input through the reference CPU OpenMP path "
+                           f"(oracle/_ref): keys/sort/tree/link/centers on all particles ({sum(t[:5]):.0f} ms), "
+                           f"neighbors for every {stride}th particle ({t[5]:.0f} ms, x{stride}); "
+                           f"mean neighbors {ncs.mean():.1f}"),
+            }
+    print(json.dumps(result), flush=True)
 
 
 def load_profile(name):
@@ -199,6 +337,7 @@ def halo_cpu_baseline(hip, n):
         "value": round(n / (total / 1e3) / 1e6, 3),
         "unit": "Mparticles/s",
         "cores": REF.omp_threads(),
+        "cpu_model": cpu_model(),
         "kind": "reference",
         "sample": (f"the same mixture workload generated at {n} particles, one pass of the whole rank-share chain "
                    f"through the reference CPU path (oracle/_ref): {total:.0f} ms, {m} local particles"),
@@ -360,7 +499,7 @@ def grav_cpu_stage_baseline(dom, d):
     cpu_ms = (time.perf_counter() - t0) * 1e3
     ok = bool(np.array_equal(c.view(np.uint8), dom.expansion_centers().view(np.uint8)))
     return {"stage": "expansion_centers", "cpu_ms": round(cpu_ms, 2), "cores": REF.omp_threads(),
-            "kind": "reference", "matches_gpu": ok}
+            "cpu_model": cpu_model(), "kind": "reference", "matches_gpu": ok}
 
 
 def ijloop_bench(args):
@@ -445,7 +584,7 @@ def ijloop_cpu_baseline(v, n, m, ngmax, stride=32):
                        centers, sizes, ngmax)
     ms = (time.time() - t0) * 1e3
     return {"value": round(last / (ms / 1e3) / 1e6, 3), "unit": "Mparticles/s", "cores": REF.omp_threads(),
-            "kind": "reference",
+            "cpu_model": cpu_model(), "kind": "reference",
             "sample": (f"the reference CPU ijloop (CpuAlwaysTraverseNeighborhood, oracle/_ref) with the same "
                        f"interaction over bodies [0, {last}) of the same sorted particles and tree: {ms:.0f} ms")}
 
@@ -459,8 +598,9 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "halo", "domain", "grav", "ijloop"], default="plain",
-                    help="plain: the 16.8M neighbor pipeline (default); halo: the 134M rank-share chain; "
+    ap.add_argument("--workload", choices=["plain", "c1", "halo", "domain", "grav", "ijloop"], default="plain",
+                    help="plain: the 16.8M neighbor pipeline (default); c1: the 1M fp64 full-build pipeline; "
+                         "halo: the 134M rank-share chain; "
                          "domain: one-rank Domain::sync + neighbors at 16.8M; grav: one-rank Domain::syncGrav at "
                          "16.8M; ijloop: the ijLoop example at 16.8M")
     ap.add_argument("--n-halo", type=int, default=134217728)
@@ -494,6 +634,9 @@ def main():
 
     if args.workload == "halo":
         halo_bench(args)
+        return
+    if args.workload == "c1":
+        c1_bench(args)
         return
     if args.workload == "ijloop":
         return ijloop_bench(args)
@@ -561,9 +704,10 @@ def main():
     achieved = nb_bytes / (nb_avg / 1e3) / 1e9
     traffic = load_traffic()
 
-    # whole-step algorithmic bytes (per particle): displace 16 read + 12 write, keys 12+8, sort 8 read + 8+4 write,
-    # gather 4+16+16, neighbors 16+4+4k
-    step_bytes = n * (28 + 20 + 20 + 36 + 20 + 4.0 * kbar)
+    # whole-step algorithmic bytes of the pipeline's stages (the synthetic displacement that moves the input between
+    # steps is inside the timed step but not counted): keys 12+8, sort 8 read + 8+4 write, gather 4+16+16, tree,
+    # link, centers and layout per leaf / node, neighbors 16+4+4k per particle
+    step_bytes = float(sum(pipeline_stage_bytes(n, int(v.num_leaves), int(v.num_nodes), kbar).values()))
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -605,8 +749,7 @@ def main():
             "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
             "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         },
-        "stages": {name: {"avg_ms": round(ms / max(c, 1), 4), "launches": c} for name, (ms, c) in stages.items()
-                   if c > 0},
+        "stages": stage_report(stages, pipeline_stage_bytes(n, int(v.num_leaves), int(v.num_nodes), kbar)),
     }
     if not args.no_cpu_baseline:
         t1 = time.time()

@@ -904,7 +904,11 @@ private:
         HC_CHECK(hipMemcpyAsync(hostFlags, flags, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, stream(s)));
         HC_CHECK(hipMemcpyAsync(hostFlags + 3, leafOps_.as<int32_t>() + L, sizeof(int32_t), hipMemcpyDeviceToHost,
                                 stream(s)));
+        // the device's fault word rides along with this synchronization (guards of the sort and earlier kernels)
+        uint32_t faults = 0;
+        if (auto e = enqueueFaultRead(stream(s), &faults)) return e;
         HC_CHECK(hipStreamSynchronize(stream(s)));
+        if (auto e = checkFaults(faults)) return e;
         const int status = hostFlags[0];
         const int32_t newL = hostFlags[3];
         int conv = hostFlags[1] == 0;

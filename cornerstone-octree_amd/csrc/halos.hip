@@ -160,6 +160,7 @@ struct HaloArgs
     typename BoxOf<T>::type box;
     int32_t firstNode, lastNode, numNodes;
     uint8_t* flags;
+    uint32_t* faults; // HIPCSTONE_FAULT_MALFORMED_TREE when a parent index leaves the tree
 };
 
 // containedIn(nodeStart, nodeEnd, lowestKey, highestKey) of the node with this prefix (boxoverlap.hpp:163-168)
@@ -213,8 +214,9 @@ __device__ inline bool chainPasses(const HaloArgs<K, T>& A, int32_t a, const T t
         if (!overlapBox(c, s, tC, tS, A.box)) return false;
         if (a == 0) return true;
         a = A.parents[(a - 1) / 8];
-        if (a < 0 || a >= A.numNodes) return false; // malformed tree
+        if (a < 0 || a >= A.numNodes) break;
     }
+    atomicOr(A.faults, HIPCSTONE_FAULT_MALFORMED_TREE); // out-of-range parent, or no root within 64 steps
     return false;
 }
 
@@ -498,8 +500,10 @@ static hipcstone_status findHalos(hipcstone_stream s, const K* prefixes, const i
         shape.count[k]  = k <= lay.top ? lay.count[k] : 0;
     }
     shape.top = lay.top;
+    uint32_t* faults = faultWord();
+    if (!faults) return HIPCSTONE_HIP_ERROR;
     HaloArgs<K, T> A{prefixes, childOffsets, parents, nodeCenters, nodeSizes, searchCenters, searchSizes,
-                     leaves,   *box,         firstNode, lastNode,  numNodes,  flags};
+                     leaves,   *box,         firstNode, lastNode,  numNodes,  flags,         faults};
 
     haloCountersInitKernel<<<1, 64, 0, stream(s)>>>(ctr, shape);
     HC_CHECK_LAUNCH("haloCountersInitKernel");

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
     const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
     const T* __restrict__ sizes, int32_t numNodes, float ext, uint32_t ngmax, uint32_t* __restrict__ neighbors,
-    uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded)
+    uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded, uint32_t* __restrict__ faults)
 {
     __shared__ NbShared<T> sh;
     const int lane       = laneId();
@@ -431,13 +431,22 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     if (rootMask != 0)
     {
         int32_t rootChild = uniform(childOffsets[0]);
+        // a guard that fires (out-of-range node, stack overflow, step budget) ends the walk and sets its bit in
+        // the fault word: the wave's rows and counts are then incomplete
+        uint32_t fault = 0;
         if (rootChild == 0) { appendLeaf(uniform(internalToLeaf[0]), rootMask); }
-        else if (rootChild > 0 && rootChild + 8 <= numNodes) // malformed tree guard
+        else if (rootChild < 0 || rootChild + 8 > numNodes) { fault = HIPCSTONE_FAULT_MALFORMED_TREE; }
+        else
         {
             int depth         = descend(0, rootChild, rootMask) ? 0 : -1;
             int64_t stepsLeft = 2 * int64_t(numNodes) + 8;
-            while (depth >= 0 && stepsLeft-- > 0)
+            while (depth >= 0)
             {
+                if (stepsLeft-- <= 0)
+                {
+                    fault = HIPCSTONE_FAULT_WALK_BUDGET;
+                    break;
+                }
                 if (listCount > kLeafList - 9) drain();
                 int k      = uniform(sh.next[w][depth]);
                 uint64_t m = 0;
@@ -455,16 +464,25 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 int32_t node = uniform(sh.co[w][depth]) + k;
                 if (lane == 0) sh.next[w][depth] = k + 1;
                 waveSync();
-                if (node <= 0 || node >= numNodes) break; // malformed tree guard
+                if (node <= 0 || node >= numNodes)
+                {
+                    fault = HIPCSTONE_FAULT_MALFORMED_TREE;
+                    break;
+                }
                 int32_t cco = uniform(childOffsets[node]);
                 if (cco == 0) { appendLeaf(uniform(internalToLeaf[node]), m); }
                 else
                 {
-                    if (depth + 1 >= kMaxDepth || cco <= node || cco + 8 > numNodes) break;
+                    if (depth + 1 >= kMaxDepth || cco <= node || cco + 8 > numNodes)
+                    {
+                        fault = HIPCSTONE_FAULT_MALFORMED_TREE;
+                        break;
+                    }
                     if (descend(depth + 1, cco, m)) ++depth;
                 }
             }
         }
+        if (fault != 0 && lane == 0) atomicOr(faults, fault);
         drain();
     }
 
@@ -490,12 +508,14 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
     size_t threads  = groups * kWave;
     unsigned blocks = ceilDiv(threads, kNbThreads);
     blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
+    uint32_t* faults = faultWord();
+    if (!faults) return HIPCSTONE_HIP_ERROR;
     bool anyPbc     = box->bnd[0] == HIPCSTONE_PERIODIC || box->bnd[1] == HIPCSTONE_PERIODIC ||
                   box->bnd[2] == HIPCSTONE_PERIODIC;
     auto launch = [&](auto pbc, auto slots) {
         findNeighborsKernel<T, decltype(pbc)::value, decltype(slots)::value><<<blocks, kNbThreads, 0, stream(s)>>>(
             x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
-            ngmax, neighbors, neighborsCount, blocks);
+            ngmax, neighbors, neighborsCount, blocks, faults);
     };
     if (anyPbc && slotMajor) launch(std::true_type{}, std::true_type{});
     else if (anyPbc) launch(std::true_type{}, std::false_type{});

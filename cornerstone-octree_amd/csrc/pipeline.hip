@@ -567,9 +567,14 @@ private:
         if (auto e = layout_.reserve((size_t(L) + 1) * sizeof(uint32_t))) return e;
         if (auto e = scanWs_.reserve(hipcstone_scan_workspace_size(size_t(L)))) return e;
         numNodes_ = N;
-        return A::build(s, tree_[treeCur_].template as<K>(), L, prefixes_.as<K>(), childOffsets_.as<int32_t>(),
-                        parents_.as<int32_t>(), levelRange_.as<int32_t>(), internalToLeaf_.as<int32_t>(),
-                        leafToInternal_.as<int32_t>(), buildWs_.ptr, buildWs_.bytes, levelRangeHost_);
+        // the device's fault word rides along with the level-range synchronization of the link
+        uint32_t faults = 0;
+        if (auto e = enqueueFaultRead(stream(s), &faults)) return e;
+        if (auto e = A::build(s, tree_[treeCur_].template as<K>(), L, prefixes_.as<K>(), childOffsets_.as<int32_t>(),
+                              parents_.as<int32_t>(), levelRange_.as<int32_t>(), internalToLeaf_.as<int32_t>(),
+                              leafToInternal_.as<int32_t>(), buildWs_.ptr, buildWs_.bytes, levelRangeHost_))
+            return e;
+        return checkFaults(faults);
     }
 
     int curve_;

@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
                                                                uint32_t* __restrict__ valuesOut, size_t n, int shift,
                                                                int digitBits, const uint32_t* __restrict__ digitOffsets,
                                                                uint32_t* __restrict__ tileCounter,
-                                                               uint32_t* __restrict__ status, GatherArgs<GT> g)
+                                                               uint32_t* __restrict__ status, GatherArgs<GT> g,
+                                                               uint32_t* __restrict__ faults, uint32_t spinLimit)
 {
     // the tile is reordered through LDS twice, keys first, then values in the same storage
     __shared__ union
@@ -345,8 +346,13 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
             if (stalled)
             {
                 // a predecessor has not published yet; every tile publishes before it waits, so this ends. The
-                // cap only guards against a corrupted workspace.
-                if (++spins > (1u << 22)) break;
+                // cap only guards against a corrupted workspace: giving up leaves a wrong prefix, which the fault
+                // word reports
+                if (++spins > spinLimit)
+                {
+                    atomicOr(faults, HIPCSTONE_FAULT_SORT_LOOKBACK);
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
@@ -479,6 +485,9 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
 
     SortWorkspace w = carveWorkspace(workspace, n, passes, pack);
     size_t tiles    = numTiles(n);
+    uint32_t* faults         = faultWord();
+    const uint32_t spinLimit = sortLookbackLimit();
+    if (!faults) return HIPCSTONE_HIP_ERROR;
     if (auto st = zeroU32(w.hist, size_t(kMaxPasses) * kRadix, s)) return st;
     if (auto st = zeroU32(w.counter, kMaxPasses + 1, s)) return st;
     if (auto st = zeroU32(w.status, size_t(passes) * tiles * kRadix, s)) return st;
@@ -533,19 +542,19 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
         bool gatherNow = last && g != nullptr;
         if (mode == 0)
             onesweepPassKernel<K, 0, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, nullptr, n, p * kRadixBits, bits, off, cnt, st, noGather);
+                src, dst, nullptr, nullptr, n, p * kRadixBits, bits, off, cnt, st, noGather, faults, spinLimit);
         else if (mode == 2 && !gatherNow)
             onesweepPassKernel<K, 2, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather);
+                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather, faults, spinLimit);
         else if (mode == 2 && gatherNow)
             onesweepPassKernel<K, 2, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, gp);
+                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, gp, faults, spinLimit);
         else if (!gatherNow)
             onesweepPassKernel<K, 1, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather);
+                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather, faults, spinLimit);
         else
             onesweepPassKernel<K, 1, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, gp);
+                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, gp, faults, spinLimit);
         HC_CHECK_LAUNCH("onesweepPassKernel");
         src  = dst;
         vsrc = vdst;

@@ -3,12 +3,79 @@
 
 #include "common.hpp"
 
+#include <atomic>
+#include <mutex>
+
 namespace hipcstone
 {
 
 static thread_local std::string g_lastError;
 
 void setError(const std::string& msg) { g_lastError = msg; }
+
+// the per-device fault word; host code uses the address hipGetSymbolAddress returns for the current device, never
+// the name
+static __device__ __align__(256) uint32_t g_faultWord[64];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_faultMutex;
+static uint32_t* g_faultPtr[kMaxDevices];
+
+uint32_t* faultWord()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    {
+        setError("fault word: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_faultMutex);
+    if (!g_faultPtr[dev])
+    {
+        void* p      = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_faultWord));
+        if (e != hipSuccess)
+        {
+            setError(std::string("fault word: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_faultPtr[dev] = static_cast<uint32_t*>(p);
+    }
+    return g_faultPtr[dev];
+}
+
+__global__ void takeFaultsKernel(uint32_t* word, uint32_t* out)
+{
+    *out  = *word;
+    *word = 0;
+}
+
+hipcstone_status enqueueFaultRead(hipStream_t s, uint32_t* host)
+{
+    uint32_t* w = faultWord();
+    if (!w) return HIPCSTONE_HIP_ERROR;
+    HC_CHECK(hipMemcpyAsync(host, w, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    return HIPCSTONE_OK;
+}
+
+hipcstone_status checkFaults(uint32_t faults)
+{
+    if (faults == 0) return HIPCSTONE_OK;
+    std::string msg = "device fault:";
+    if (faults & HIPCSTONE_FAULT_SORT_LOOKBACK) msg += " radix sort look-back gave up;";
+    if (faults & HIPCSTONE_FAULT_WALK_BUDGET) msg += " neighbor walk exceeded its step budget;";
+    if (faults & HIPCSTONE_FAULT_MALFORMED_TREE) msg += " tree walk met an out-of-range node;";
+    setError(msg);
+    return HIPCSTONE_DEVICE_FAULT;
+}
+
+static std::atomic<uint32_t> g_lookbackLimit{0};
+
+uint32_t sortLookbackLimit()
+{
+    uint32_t v = g_lookbackLimit.load();
+    return v ? v : (1u << 22);
+}
 
 __global__ void fillBytesKernel(uint8_t* dst, uint8_t value, size_t bytes)
 {
@@ -57,6 +124,26 @@ extern "C"
 {
 
 const char* hipcstone_last_error(void) { return g_lastError.c_str(); }
+
+hipcstone_status hipcstone_device_faults(hipcstone_stream s, uint32_t* faults)
+{
+    uint32_t* w = faultWord();
+    if (!w) return HIPCSTONE_HIP_ERROR;
+    // a one-word staging slot next to the word itself: read and clear it in one kernel, then copy the slot out
+    takeFaultsKernel<<<1, 1, 0, stream(s)>>>(w, w + 32);
+    HC_CHECK_LAUNCH("takeFaultsKernel");
+    uint32_t v = 0;
+    HC_CHECK(hipMemcpyAsync(&v, w + 32, sizeof(v), hipMemcpyDeviceToHost, stream(s)));
+    HC_CHECK(hipStreamSynchronize(stream(s)));
+    if (faults) *faults = v;
+    return checkFaults(v);
+}
+
+hipcstone_status hipcstone_set_sort_lookback_limit(uint32_t polls)
+{
+    g_lookbackLimit.store(polls);
+    return HIPCSTONE_OK;
+}
 
 const char* hipcstone_build_info(void) { return "hipcstone 0.1 (gfx950, wave64, -ffp-contract=off)"; }
 

@@ -28,6 +28,16 @@ inline hipcstone_status invalid(const char* what)
 
 inline hipStream_t stream(hipcstone_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
+// The current device's fault word (device address): kernels whose guards fire atomically OR HIPCSTONE_FAULT_* bits
+// into it. Resolved once per device; nullptr if the symbol lookup failed (the error text is then set).
+uint32_t* faultWord();
+// Read back with a stream-ordered copy into host memory (no synchronization); the orchestration objects enqueue it
+// before a synchronization they make anyway and then call checkFaults().
+hipcstone_status enqueueFaultRead(hipStream_t s, uint32_t* host);
+hipcstone_status checkFaults(uint32_t faults);
+// the radix sort's look-back poll limit (hipcstone_set_sort_lookback_limit)
+uint32_t sortLookbackLimit();
+
 } // namespace hipcstone
 
 // Return the status of a failed HIP call from the enclosing C ABI function.

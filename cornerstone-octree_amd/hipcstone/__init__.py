@@ -35,7 +35,9 @@ HILBERT = 1
 MORTON = 0
 OPEN, PERIODIC, FIXED, CUBIC_OPEN = 0, 1, 2, 3
 
-STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP_ERROR", 4: "CAPACITY"}
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP_ERROR", 4: "CAPACITY", 5: "DEVICE_FAULT"}
+DEVICE_FAULT = 5
+FAULT_SORT_LOOKBACK, FAULT_WALK_BUDGET, FAULT_MALFORMED_TREE = 1, 2, 4
 
 _lib.hipcstone_last_error.restype = C.c_char_p
 _lib.hipcstone_build_info.restype = C.c_char_p
@@ -55,6 +57,21 @@ def _check(status, where):
 
 def build_info() -> str:
     return _lib.hipcstone_build_info().decode()
+
+
+def device_faults(stream=None):
+    """Synchronizes the stream, reads and clears the device's fault word; returns its HIPCSTONE_FAULT_* bits (0 when
+    no kernel guard fired)."""
+    v = C.c_uint32()
+    st = _lib.hipcstone_device_faults(_stream(stream), C.byref(v))
+    if st not in (0, DEVICE_FAULT):
+        _check(st, "device_faults")
+    return int(v.value)
+
+
+def set_sort_lookback_limit(polls):
+    """Test hook: polls a radix sort tile makes of an unpublished predecessor before giving up (0 = default)."""
+    _check(_lib.hipcstone_set_sort_lookback_limit(C.c_uint32(int(polls))), "set_sort_lookback_limit")
 
 
 # ---------------------------------------------------------------- boxes

@@ -29,8 +29,16 @@ typedef enum
     HIPCSTONE_INVALID_ARG   = 1, /* sizes or pointers inconsistent; the reference throws std::runtime_error */
     HIPCSTONE_OUT_OF_MEMORY = 2, /* a hipcstone_*_alloc call failed */
     HIPCSTONE_HIP_ERROR     = 3, /* a HIP runtime call failed; see hipcstone_last_error() */
-    HIPCSTONE_CAPACITY      = 4  /* an output buffer is smaller than the result; the required size is returned */
+    HIPCSTONE_CAPACITY      = 4, /* an output buffer is smaller than the result; the required size is returned */
+    HIPCSTONE_DEVICE_FAULT  = 5  /* a kernel's guard fired: its output is incomplete (hipcstone_device_faults) */
 } hipcstone_status;
+
+/* Guards in the kernels that keep every loop bounded. When one fires the kernel still ends normally, its output is
+ * not valid, and it sets its bit in the device's fault word; hipcstone_device_faults reads and clears that word.
+ * The reference has no counterpart (its kernels spin or walk unboundedly on such input). */
+#define HIPCSTONE_FAULT_SORT_LOOKBACK  1u /* a radix sort tile gave up waiting for its predecessor's prefix */
+#define HIPCSTONE_FAULT_WALK_BUDGET    2u /* a neighbor walk used up its step budget (a cycle in child offsets) */
+#define HIPCSTONE_FAULT_MALFORMED_TREE 4u /* a tree walk met a node index outside the tree, or a stack overflow */
 
 /* curve kinds for key encoding (sfc/sfc.hpp:30-40 MortonKey / HilbertKey) */
 enum
@@ -75,6 +83,13 @@ typedef void* hipcstone_stream; /* a hipStream_t */
 
 /* text of the last error on the calling host thread (cuda/errorcheck.cuh:15-27 prints and exits instead) */
 const char* hipcstone_last_error(void);
+/* Synchronizes s, then reads and clears the fault word of the current device (HIPCSTONE_FAULT_* bits of every guard
+ * that fired since the last call, on any stream; *faults may be NULL). Returns HIPCSTONE_DEVICE_FAULT if any bit was
+ * set. The pipeline and domain objects also check the word at their own synchronization points. */
+hipcstone_status hipcstone_device_faults(hipcstone_stream s, uint32_t* faults);
+/* Test hook: the number of polls a radix sort tile makes of a predecessor that has not published its prefix before
+ * it gives up (HIPCSTONE_FAULT_SORT_LOOKBACK); 0 restores the default (2^22). Applies to later sort launches. */
+hipcstone_status hipcstone_set_sort_lookback_limit(uint32_t polls);
 /* library version and the gfx target it was built for */
 const char* hipcstone_build_info(void);
 

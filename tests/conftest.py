@@ -39,6 +39,18 @@ def hip():
     return hipcstone
 
 
+@pytest.fixture(autouse=True)
+def _no_device_faults(request):
+    """Every GPU test ends with the device's fault word clear: no kernel guard fired (a test that provokes one reads
+    and clears the word itself)."""
+    yield
+    if request.node.get_closest_marker("gpu") is not None:
+        import hipcstone
+
+        faults = hipcstone.device_faults()
+        assert faults == 0, f"kernel guards fired during the test: fault bits {faults:#x}"
+
+
 def golden_path(name):
     return os.path.join(ROOT, "tests", "golden", name)
 
