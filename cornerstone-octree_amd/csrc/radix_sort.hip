@@ -393,7 +393,9 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
             K key        = shTile.keys[i];
             unsigned d   = unsigned(key >> shift) & digitMask;
             dsts[m]      = shGlobalBase[d] + i; // modulo 2^32: the base alone may wrap below zero
-            keysOut[dsts[m]] = key;
+            // a correct prefix always lands inside [0, n); after a look-back give-up (reported through the fault
+            // word) later passes see digit counts that disagree with the histogram, and this keeps them in bounds
+            if (dsts[m] < n) keysOut[dsts[m]] = key;
         }
     }
     if constexpr (ValueMode == 0) return;
@@ -416,9 +418,11 @@ __global__ __launch_bounds__(kThreads) void onesweepPassKernel(const K* __restri
         {
             const uint32_t dst = dsts[m];
             uint32_t v         = shTile.vals[i];
+            if (dst >= n) continue; // only after a look-back give-up (see above)
             valuesOut[dst]     = v;
             if constexpr (GatherOn)
             {
+                if (v >= n) continue; // gather values are input positions; see above
                 if (usePacked)
                 {
                     using V4 = typename std::conditional<sizeof(GT) == 4, float4, double4>::type;
