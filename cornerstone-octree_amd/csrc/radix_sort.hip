@@ -21,6 +21,9 @@
 // choice.
 #include <type_traits>
 
+#include <mutex>
+#include <string>
+
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -593,12 +596,64 @@ static hipcstone_status sortGather(hipcstone_stream s, K* keys, uint32_t* perm, 
     return radixSort<K, T>(stream(s), keys, perm, n, keyBuf, valueBuf, ws, wsBytes, int(8 * sizeof(K)), 2, &g);
 }
 
+// ---- the no-allocation sort: its workspace lives in a file-scope device array of fixed size, sized for the
+// largest accepted key count of 64-bit keys with all 8 passes. Each device holds its own copy of the array; host
+// code never uses the array's name as an address, only the per-device pointer hipGetSymbolAddress returned.
+constexpr size_t kNoAllocMaxKeys = size_t(1) << 24;
+constexpr size_t kNoAllocScratchBytes =
+    ((sizeof(uint32_t) * kMaxPasses * kRadix + 255) & ~size_t(255)) + ((sizeof(uint32_t) * (kMaxPasses + 1) + 255) &
+                                                                        ~size_t(255)) +
+    ((sizeof(uint32_t) * size_t(kMaxPasses) * ((kNoAllocMaxKeys + kTile - 1) / kTile) * kRadix + 255) & ~size_t(255));
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratchMutex;
+static void* g_scratchPtr[kNoAllocMaxDevices];
+
+// the scratch array's address on the current device, looked up once per device with that device current
+static void* noAllocScratch()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+    {
+        setError("sort (no allocation): no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev])
+    {
+        void* ptr    = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+        {
+            setError(std::string("sort (no allocation): hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratchPtr[dev] = ptr;
+    }
+    return g_scratchPtr[dev];
+}
+
 } // namespace hipcstone
 
 using namespace hipcstone;
 
 extern "C"
 {
+
+hipcstone_status hipcstone_sort_by_key_u64_noalloc(hipcstone_stream s, uint64_t* keys, uint32_t* values, size_t n,
+                                                   uint64_t* key_buf, uint32_t* value_buf, int end_bit)
+{
+    if (n > kNoAllocMaxKeys)
+        return invalid("sort_by_key_u64_noalloc: more than 2^24 keys (the static scratch's maximum)");
+    void* ptr = noAllocScratch();
+    if (!ptr) return HIPCSTONE_HIP_ERROR;
+    static_assert(kNoAllocScratchBytes >= 8192, "scratch holds at least the histograms");
+    // every part of the workspace that must start at zero is cleared by radixSort itself (fill kernels)
+    return radixSort<uint64_t, float>(stream(s), keys, values, n, key_buf, value_buf, ptr, kNoAllocScratchBytes,
+                                      end_bit, values ? 1 : 0, nullptr);
+}
 
 size_t hipcstone_sort_by_key_workspace_size_u32(size_t n) { return sortWorkspaceSize<uint32_t>(n, false); }
 size_t hipcstone_sort_by_key_workspace_size_u64(size_t n) { return sortWorkspaceSize<uint64_t>(n, false); }

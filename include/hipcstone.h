@@ -145,6 +145,15 @@ hipcstone_status hipcstone_sort_by_key_u32(hipcstone_stream s, uint32_t* keys, u
 hipcstone_status hipcstone_sort_by_key_u64(hipcstone_stream s, uint64_t* keys, uint32_t* values, size_t n,
                                            uint64_t* key_buf, uint32_t* value_buf, void* workspace,
                                            size_t workspace_bytes, int end_bit);
+/* No counterpart in the reference: hipcstone_sort_by_key_u64 without the workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Its workspace is a static device array of the library (about 22.3 MiB per device, held until the
+ * process ends), so it accepts at most 2^24 keys (HIPCSTONE_INVALID_ARG beyond) and allocates nothing. Every call on
+ * one device shares that device's array: calls that overlap on one device (other streams, other host threads) race
+ * and must be serialized by the caller. key_buf/value_buf, values == NULL and end_bit as for
+ * hipcstone_sort_by_key_u64. */
+hipcstone_status hipcstone_sort_by_key_u64_noalloc(hipcstone_stream s, uint64_t* keys, uint32_t* values, size_t n,
+                                                   uint64_t* key_buf, uint32_t* value_buf, int end_bit);
 
 /* ---- a4+a5 fused: keys sorted with the permutation iota(0..n) and x,y,z,h gathered by it, the sequence of
  * sequence() + sortByKey() + gather() x4 (primitives/primitives_gpu.h:100-155, :48-56; domain/domain.hpp:187-188).
