@@ -108,6 +108,30 @@ __host__ __device__ inline unsigned octalDigit(K code, unsigned position)
     return unsigned((code >> (3u * (maxLevel<K>() - position))) & 7u);
 }
 
+// tree/octree.hpp:55-65 with sfc/common.hpp:318-322 digitWeight: the weight that places the internal node created
+// between two adjacent leaves at its slot (tid + weight) / 7 of the unsorted internal-node layout
+template<class K>
+__host__ __device__ inline int32_t binaryKeyWeight(K key, unsigned level)
+{
+    int32_t ret = 0;
+    for (unsigned l = 1; l <= level + 1; ++l)
+    {
+        int digit       = int(octalDigit(key, l));
+        int fourGeqMask = -int(digit >= 4);
+        ret += ((7 - digit) & fourGeqMask) - (digit & ~fourGeqMask);
+    }
+    return ret;
+}
+
+// splitmix64 finalizer: the counter-based hash of the synthetic input generators (displacements, mixtures)
+__host__ __device__ inline uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 // sfc/common.hpp:326-331
 template<class K>
 __host__ __device__ inline K enclosingBoxCode(K key, unsigned level)

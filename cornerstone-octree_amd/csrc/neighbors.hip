@@ -101,24 +101,47 @@ struct NbShared
 
 // SlotMajor: the wave's 64 rows are stored slot-major (neighbor k of the wave's lane l at k * 64 + l) instead of
 // row-major (l * ngmax + k), so that a consumer with one thread per target reads each slot with one coalesced load.
-template<class T, bool AnyPbc, bool SlotMajor>
+// Segmented: many independent trees in one set of arrays (the batched clouds). Wave `group` then takes its targets
+// [first, last) (at most 64, inside one cloud) and the index of its cloud's root node from the wave table
+// (waves[3 g], waves[3 g + 1], cloud = waves[3 g + 2], root = roots[cloud]); rows and counts are indexed by the
+// particle index itself (firstId = 0), and a child block may lie before its parent (the batched link places children
+// by internal-node slot, not level by level), so only the bounds and depth guards and the step budget apply.
+template<class T, bool AnyPbc, bool SlotMajor, bool Segmented = false>
 __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, const T* __restrict__ h,
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
     const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
     const T* __restrict__ sizes, int32_t numNodes, float ext, uint32_t ngmax, uint32_t* __restrict__ neighbors,
-    uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded, uint32_t* __restrict__ faults)
+    uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded, uint32_t* __restrict__ faults,
+    const uint32_t* __restrict__ waves = nullptr, uint32_t numWaves = 0, const int32_t* __restrict__ roots = nullptr)
 {
     __shared__ NbShared<T> sh;
     const int lane       = laneId();
     const int w          = threadIdx.x / kWave;
     const uint32_t block = xcdContiguousBlock(blockIdx.x, numBlocksPadded);
     const uint32_t group = uniform(block * uint32_t(kNbWaves) + uint32_t(w));
-    const uint64_t first = uint64_t(firstId) + uint64_t(group) * kWave;
-    if (first >= lastId) return;
+    uint64_t first;
+    int32_t root = 0;
+    if constexpr (Segmented)
+    {
+        if (group >= numWaves) return;
+        first  = uniform(waves[3 * group]);
+        lastId = uniform(waves[3 * group + 1]);
+        root   = uniform(roots[uniform(waves[3 * group + 2])]);
+        if (root < 0 || root >= numNodes)
+        {
+            if (lane == 0) atomicOr(faults, HIPCSTONE_FAULT_MALFORMED_TREE);
+            return;
+        }
+    }
+    else
+    {
+        first = uint64_t(firstId) + uint64_t(group) * kWave;
+        if (first >= lastId) return;
+    }
     const uint32_t i     = uint32_t(first) + uint32_t(lane);
     const bool valid     = i < lastId;
-    const size_t rowBase = size_t(group) * kWave; // row of lane 0 in the output
+    const size_t rowBase = Segmented ? size_t(first) : size_t(group) * kWave; // row of lane 0 in the output
 
     T xi = 0, yi = 0, zi = 0, hi = 0;
     if (valid)
@@ -426,15 +449,16 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     };
 
     // root (traversal.hpp:35-45)
-    uint64_t rootMask = ballot(valid & (nodeDist2(centers[0], centers[1], centers[2], sizes[0], sizes[1], sizes[2]) <
-                                        cellR2));
+    const T* rc       = centers + size_t(root) * 3;
+    const T* rsz      = sizes + size_t(root) * 3;
+    uint64_t rootMask = ballot(valid & (nodeDist2(rc[0], rc[1], rc[2], rsz[0], rsz[1], rsz[2]) < cellR2));
     if (rootMask != 0)
     {
-        int32_t rootChild = uniform(childOffsets[0]);
+        int32_t rootChild = uniform(childOffsets[root]);
         // a guard that fires (out-of-range node, stack overflow, step budget) ends the walk and sets its bit in
         // the fault word: the wave's rows and counts are then incomplete
         uint32_t fault = 0;
-        if (rootChild == 0) { appendLeaf(uniform(internalToLeaf[0]), rootMask); }
+        if (rootChild == 0) { appendLeaf(uniform(internalToLeaf[root]), rootMask); }
         else if (rootChild < 0 || rootChild + 8 > numNodes) { fault = HIPCSTONE_FAULT_MALFORMED_TREE; }
         else
         {
@@ -473,7 +497,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
                 if (cco == 0) { appendLeaf(uniform(internalToLeaf[node]), m); }
                 else
                 {
-                    if (depth + 1 >= kMaxDepth || cco <= node || cco + 8 > numNodes)
+                    if (depth + 1 >= kMaxDepth || (!Segmented && cco <= node) || cco <= 0 || cco + 8 > numNodes)
                     {
                         fault = HIPCSTONE_FAULT_MALFORMED_TREE;
                         break;
@@ -488,6 +512,34 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
 
     if (valid) neighborsCount[i - firstId] = SlotMajor ? (myWpos - rowOff) >> 6 : myWpos - rowOff;
 }
+
+// the segmented search of the batched clouds (row-major rows, open boxes): see findNeighborsKernel
+template<class T>
+hipcstone_status findNeighborsSegmented(hipStream_t s, const T* x, const T* y, const T* z, const T* h,
+                                        const typename BoxOf<T>::type& box, const int32_t* childOffsets,
+                                        const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,
+                                        const T* sizes, int32_t numNodesCap, uint32_t ngmax, uint32_t* neighbors,
+                                        uint32_t* neighborsCount, const uint32_t* waves, uint32_t numWaves,
+                                        const int32_t* roots)
+{
+    if (numWaves == 0) return HIPCSTONE_OK;
+    if (ngmax > (1u << 20)) return invalid("findNeighbors: ngmax above 2^20");
+    uint32_t* faults = faultWord();
+    if (!faults) return HIPCSTONE_HIP_ERROR;
+    unsigned blocks = ceilDiv(size_t(numWaves), kNbWaves);
+    blocks          = (blocks + kXcds - 1) / kXcds * kXcds;
+    findNeighborsKernel<T, false, false, true><<<blocks, kNbThreads, 0, s>>>(
+        x, y, z, h, 0u, 0u, box, childOffsets, internalToLeaf, layout, centers, sizes, numNodesCap, 1.0f, ngmax,
+        neighbors, neighborsCount, blocks, faults, waves, numWaves, roots);
+    HC_CHECK_LAUNCH("findNeighborsKernel (segmented)");
+    return HIPCSTONE_OK;
+}
+
+template hipcstone_status findNeighborsSegmented<float>(hipStream_t, const float*, const float*, const float*,
+                                                        const float*, const BoxOf<float>::type&, const int32_t*,
+                                                        const int32_t*, const uint32_t*, const float*, const float*,
+                                                        int32_t, uint32_t, uint32_t*, uint32_t*, const uint32_t*,
+                                                        uint32_t, const int32_t*);
 
 template<class T>
 static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h,

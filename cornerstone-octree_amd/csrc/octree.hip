@@ -14,20 +14,6 @@ extern "C" size_t hipcstone_sort_by_key_workspace_size_u64(size_t n);
 namespace hipcstone
 {
 
-// tree/octree.hpp:55-65 with sfc/common.hpp:318-322 digitWeight
-template<class K>
-__device__ inline int32_t binaryKeyWeight(K key, unsigned level)
-{
-    int32_t ret = 0;
-    for (unsigned l = 1; l <= level + 1; ++l)
-    {
-        int digit       = int(octalDigit(key, l));
-        int fourGeqMask = -int(digit >= 4);
-        ret += ((7 - digit) & fourGeqMask) - (digit & ~fourGeqMask);
-    }
-    return ret;
-}
-
 template<class K>
 __global__ __launch_bounds__(256) void unsortedLayoutKernel(const K* __restrict__ leaves, int32_t numInternal,
                                                             int32_t numLeaves, K* __restrict__ prefixes,

@@ -26,6 +26,7 @@
 
 #include "common.hpp"
 #include "runtime.hpp"
+#include "sort.hpp"
 
 namespace hipcstone
 {
@@ -105,7 +106,8 @@ __device__ inline void atomicStoreRelaxed(uint32_t* p, uint32_t v)
 // With descents set, also counts the positions i with keys[i] > keys[i + 1].
 template<class K>
 __global__ __launch_bounds__(kThreads) void histogramKernel(const K* __restrict__ keys, size_t n, int numPasses,
-                                                            uint32_t* __restrict__ hist, uint32_t* __restrict__ descents)
+                                                            int beginBit, int endBit, uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ descents)
 {
     __shared__ uint32_t sh[kMaxPasses * kRadix];
     __shared__ uint32_t shDescents;
@@ -131,7 +133,10 @@ __global__ __launch_bounds__(kThreads) void histogramKernel(const K* __restrict_
         int first = __builtin_ctzll(act);
         for (int p = 0; p < numPasses; ++p)
         {
-            unsigned d      = unsigned(key >> (p * kRadixBits)) & (kRadix - 1);
+            // the digit of pass p as the pass kernel reads it: bits [shift, min(shift + 8, endBit))
+            const int shift = beginBit + p * kRadixBits;
+            const int bits  = endBit - shift < kRadixBits ? endBit - shift : kRadixBits;
+            unsigned d      = unsigned(key >> shift) & ((1u << bits) - 1u);
             unsigned d0     = unsigned(__builtin_amdgcn_readlane(int(d), first));
             uint64_t same   = ballot(valid && d == d0);
             if (same == act)
@@ -478,12 +483,14 @@ static int numPassesFor(int endBit)
 // Generic driver. valueMode 0/1/2 as in onesweepPassKernel. When gather is set, the last pass gathers.
 template<class K, class GT>
 static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size_t n, K* keyBuf, uint32_t* valueBuf,
-                                  void* workspace, size_t wsBytes, int endBit, int valueMode, const GatherArgs<GT>* g)
+                                  void* workspace, size_t wsBytes, int endBit, int valueMode, const GatherArgs<GT>* g,
+                                  int beginBit = 0)
 {
     if (n == 0) return HIPCSTONE_OK;
     if (n > kValueMask) return invalid("radix sort: more than 2^30-1 keys");
     if (endBit <= 0 || endBit > int(8 * sizeof(K))) endBit = int(8 * sizeof(K));
-    int passes = numPassesFor<K>(endBit);
+    if (beginBit < 0 || beginBit >= endBit) return invalid("radix sort: begin bit outside [0, end bit)");
+    int passes = numPassesFor<K>(endBit - beginBit);
     const bool pack = g != nullptr;
     size_t need     = workspaceBytes(n, passes, pack);
     if (!workspace || wsBytes < need) return invalid("radix sort: workspace too small");
@@ -501,7 +508,7 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
 
     unsigned hb = unsigned(std::min<size_t>(kHistBlocks, (n + kThreads - 1) / kThreads));
     uint32_t* descents = pack ? w.counter + kMaxPasses : nullptr;
-    histogramKernel<K><<<hb, kThreads, 0, s>>>(keys, n, passes, w.hist, descents);
+    histogramKernel<K><<<hb, kThreads, 0, s>>>(keys, n, passes, beginBit, endBit, w.hist, descents);
     HC_CHECK_LAUNCH("histogramKernel");
     GatherArgs<GT> gp{};
     if (pack)
@@ -530,7 +537,8 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
         bool toBuf = ((passes - 1 - p) % 2) == 1;
         dst        = toBuf ? keyBuf : keys;
         vdst       = toBuf ? valueBuf : values;
-        int bits   = std::min(kRadixBits, endBit - p * kRadixBits);
+        int shift  = beginBit + p * kRadixBits;
+        int bits   = std::min(kRadixBits, endBit - shift);
         int mode   = (p == 0 && valueMode == 2) ? 2 : (valueMode == 0 ? 0 : 1);
         const uint32_t* off = w.hist + p * kRadix;
         uint32_t* cnt       = w.counter + p;
@@ -549,19 +557,19 @@ static hipcstone_status radixSort(hipStream_t s, K* keys, uint32_t* values, size
         bool gatherNow = last && g != nullptr;
         if (mode == 0)
             onesweepPassKernel<K, 0, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, nullptr, n, p * kRadixBits, bits, off, cnt, st, noGather, faults, spinLimit);
+                src, dst, nullptr, nullptr, n, shift, bits, off, cnt, st, noGather, faults, spinLimit);
         else if (mode == 2 && !gatherNow)
             onesweepPassKernel<K, 2, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather, faults, spinLimit);
+                src, dst, nullptr, vdst, n, shift, bits, off, cnt, st, noGather, faults, spinLimit);
         else if (mode == 2 && gatherNow)
             onesweepPassKernel<K, 2, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, nullptr, vdst, n, p * kRadixBits, bits, off, cnt, st, gp, faults, spinLimit);
+                src, dst, nullptr, vdst, n, shift, bits, off, cnt, st, gp, faults, spinLimit);
         else if (!gatherNow)
             onesweepPassKernel<K, 1, false, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, noGather, faults, spinLimit);
+                src, dst, vsrc, vdst, n, shift, bits, off, cnt, st, noGather, faults, spinLimit);
         else
             onesweepPassKernel<K, 1, true, GT><<<unsigned(tiles), kThreads, 0, s>>>(
-                src, dst, vsrc, vdst, n, p * kRadixBits, bits, off, cnt, st, gp, faults, spinLimit);
+                src, dst, vsrc, vdst, n, shift, bits, off, cnt, st, gp, faults, spinLimit);
         HC_CHECK_LAUNCH("onesweepPassKernel");
         src  = dst;
         vsrc = vdst;
@@ -584,6 +592,29 @@ static hipcstone_status gather(hipcstone_stream s, const uint32_t* perm, size_t 
     HC_CHECK_LAUNCH("gatherKernel");
     return HIPCSTONE_OK;
 }
+
+template<class K>
+size_t sortGatherWorkspaceBytes(size_t n)
+{
+    return sortWorkspaceSize<K>(n, true);
+}
+
+template<class K, class T>
+hipcstone_status sortGatherBits(hipStream_t s, K* keys, uint32_t* perm, size_t n, const T* xIn, const T* yIn,
+                                const T* zIn, const T* hIn, T* xOut, T* yOut, T* zOut, T* hOut, K* keyBuf,
+                                uint32_t* valueBuf, void* ws, size_t wsBytes, int beginBit, int endBit)
+{
+    if (n == 0) return HIPCSTONE_OK;
+    if (!xIn || !yIn || !zIn || !xOut || !yOut || !zOut || (hIn && !hOut)) return invalid("sortGather: null array");
+    GatherArgs<T> g{xIn, yIn, zIn, hIn, xOut, yOut, zOut, hOut, nullptr, nullptr, 0};
+    return radixSort<K, T>(s, keys, perm, n, keyBuf, valueBuf, ws, wsBytes, endBit, 2, &g, beginBit);
+}
+
+template size_t sortGatherWorkspaceBytes<uint64_t>(size_t);
+template hipcstone_status sortGatherBits<uint64_t, float>(hipStream_t, uint64_t*, uint32_t*, size_t, const float*,
+                                                          const float*, const float*, const float*, float*, float*,
+                                                          float*, float*, uint64_t*, uint32_t*, void*, size_t, int,
+                                                          int);
 
 template<class K, class T>
 static hipcstone_status sortGather(hipcstone_stream s, K* keys, uint32_t* perm, size_t n, const T* xIn, const T* yIn,

@@ -65,6 +65,7 @@ hipcstone_status checkFaults(uint32_t faults)
     if (faults & HIPCSTONE_FAULT_SORT_LOOKBACK) msg += " radix sort look-back gave up;";
     if (faults & HIPCSTONE_FAULT_WALK_BUDGET) msg += " neighbor walk exceeded its step budget;";
     if (faults & HIPCSTONE_FAULT_MALFORMED_TREE) msg += " tree walk met an out-of-range node;";
+    if (faults & HIPCSTONE_FAULT_CAPACITY) msg += " batched leaf array exceeded its capacity;";
     setError(msg);
     return HIPCSTONE_DEVICE_FAULT;
 }

@@ -6,17 +6,10 @@
 
 #include "common.hpp"
 #include "runtime.hpp"
+#include "synth.hpp"
 
 namespace hipcstone
 {
-
-__device__ inline uint64_t mix64(uint64_t z)
-{
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
 
 template<class T>
 __global__ __launch_bounds__(256) void displaceKernel(T* __restrict__ x, T* __restrict__ y, T* __restrict__ z,
@@ -25,19 +18,7 @@ __global__ __launch_bounds__(256) void displaceKernel(T* __restrict__ x, T* __re
 {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t base = mix64(seed * 0x100000001B3ull + step);
-    T* c[3]       = {x, y, z};
-    T amp         = scale * h[i];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        uint64_t r = mix64(base + 3 * uint64_t(i) + d);
-        T u        = T(double(r >> 11) * (1.0 / 9007199254740992.0)) * T(2) - T(1);
-        T v        = c[d][i] + amp * u;
-        T lo = box.lim[2 * d], hi = box.lim[2 * d + 1];
-        v       = v < lo ? lo : (v > hi ? hi : v);
-        c[d][i] = v;
-    }
+    displaceAt(x, y, z, h, i, uint64_t(i), scale, seed, step, box);
 }
 
 // value k of stream `stream` under `seed`, in [0, 1): the counter-based splitmix64 sequence of the host-side input

@@ -680,6 +680,117 @@ class Pipeline:
             pass
 
 
+# ---------------------------------------------------------------- batched clouds
+
+
+class BatchedView(C.Structure):
+    _fields_ = [
+        ("num_clouds", C.c_uint32),
+        ("num_waves", C.c_uint32),
+        ("num_particles", C.c_size_t),
+        ("num_leaves", C.c_int32),
+        ("num_nodes", C.c_int32),
+        ("leaf_capacity", C.c_int32),
+    ] + [(name, C.c_void_p) for name in (
+        "keys", "perm", "x", "y", "z", "h", "leaves", "counts", "layout", "prefixes", "child_offsets",
+        "internal_to_leaf", "centers", "sizes", "cloud_roots", "neighbors", "neighbor_counts")]
+
+
+CLOUD_SHIFT, LOCAL_SHIFT = 54, 24
+
+
+class Batched:
+    """Many fp32 clouds through the pipeline at once (hipcstone_batched_*): each stage one launch over all clouds,
+    warm steps without host synchronization (capturable in one HIP graph). Cloud c is bit-identical to a plain
+    32-bit-key pipeline on that cloud alone."""
+
+    def __init__(self, counts, boxes, buckets, curve=MORTON, ngmax=128, leaf_capacity_factor=0.5):
+        counts = np.ascontiguousarray(counts, dtype=np.uint64)
+        buckets = np.ascontiguousarray(buckets, dtype=np.uint32)
+        C_ = len(counts)
+        barr = (BoxF32 * C_)(*boxes)
+        self.handle = C.c_void_p()
+        self.counts = counts
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.ngmax = ngmax
+        _check(_lib.hipcstone_batched_create(C.byref(self.handle), C.c_int(curve), C.c_uint32(C_),
+                                             counts.ctypes.data_as(C.c_void_p), barr,
+                                             buckets.ctypes.data_as(C.c_void_p), C.c_uint32(ngmax),
+                                             C.c_double(leaf_capacity_factor)), "batched_create")
+
+    def set_particles(self, x, y, z, h, stream=None):
+        """x, y, z, h of all clouds in cloud order: numpy arrays (host) or DeviceArrays."""
+        dev = isinstance(x, DeviceArray)
+        if dev:
+            ptrs = [_p(a) for a in (x, y, z, h)]
+        else:
+            arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z, h)]
+            ptrs = [a.ctypes.data_as(C.c_void_p) for a in arrs]
+        _check(_lib.hipcstone_batched_set_particles(self.handle, _stream(stream), *ptrs, C.c_int(int(dev))),
+               "batched_set_particles")
+
+    def build(self, stream=None):
+        _check(_lib.hipcstone_batched_build(self.handle, _stream(stream)), "batched_build")
+
+    def step(self, drift_scale=0.0, seed=0, step=0, stream=None):
+        _check(_lib.hipcstone_batched_step(self.handle, _stream(stream), C.c_float(drift_scale), C.c_uint64(seed),
+                                           C.c_uint64(step)), "batched_step")
+
+    def capture(self, steps, drift_scale, seed, first_step, stream):
+        _check(_lib.hipcstone_batched_capture(self.handle, _stream(stream), C.c_int32(steps), C.c_float(drift_scale),
+                                              C.c_uint64(seed), C.c_uint64(first_step)), "batched_capture")
+
+    def replay(self, stream):
+        _check(_lib.hipcstone_batched_replay(self.handle, _stream(stream)), "batched_replay")
+
+    def view(self, stream=None) -> BatchedView:
+        v = BatchedView()
+        _check(_lib.hipcstone_batched_get_view(self.handle, _stream(stream), C.byref(v)), "batched_get_view")
+        return v
+
+    def device_bytes(self):
+        _lib.hipcstone_batched_device_bytes.restype = C.c_size_t
+        return int(_lib.hipcstone_batched_device_bytes(self.handle))
+
+    def arrays(self, with_neighbors=True):
+        v = self.view()
+        n, L = v.num_particles, v.num_leaves
+        out = dict(keys=download_ptr(v.keys, n, np.uint64), perm=download_ptr(v.perm, n, np.uint32),
+                   x=download_ptr(v.x, n, np.float32), y=download_ptr(v.y, n, np.float32),
+                   z=download_ptr(v.z, n, np.float32), h=download_ptr(v.h, n, np.float32),
+                   leaves=download_ptr(v.leaves, L + 1, np.uint64), counts=download_ptr(v.counts, L, np.uint32),
+                   layout=download_ptr(v.layout, L + 1, np.uint32),
+                   roots=download_ptr(v.cloud_roots, v.num_clouds, np.int32),
+                   counts_nb=download_ptr(v.neighbor_counts, n, np.uint32))
+        if with_neighbors and self.ngmax:
+            out["neighbors"] = download_ptr(v.neighbors, n * self.ngmax, np.uint32).reshape(n, self.ngmax)
+        return out
+
+    def cloud(self, a, c):
+        """Cloud c's part of arrays() in the cloud's own terms (what a plain pipeline on the cloud alone returns):
+        32-bit keys and leaves, local permutation, layout and neighbor indices."""
+        lo, hi = int(self.offsets[c]), int(self.offsets[c + 1])
+        sel = (a["leaves"][:-1] >> np.uint64(CLOUD_SHIFT)) == np.uint64(c)
+        li = np.nonzero(sel)[0]
+        leaves = ((a["leaves"][li] >> np.uint64(LOCAL_SHIFT)) & np.uint64((1 << 30) - 1)).astype(np.uint32)
+        leaves = np.concatenate([leaves, [np.uint32(1 << 30)]]).astype(np.uint32)
+        out = dict(keys=((a["keys"][lo:hi] >> np.uint64(LOCAL_SHIFT)) & np.uint64((1 << 30) - 1)).astype(np.uint32),
+                   perm=(a["perm"][lo:hi].astype(np.int64) - lo).astype(np.uint32),
+                   x=a["x"][lo:hi], y=a["y"][lo:hi], z=a["z"][lo:hi], h=a["h"][lo:hi], leaves=leaves,
+                   counts=a["counts"][li],
+                   layout=np.concatenate([a["layout"][li], [a["layout"][li[-1] + 1]]]).astype(np.int64) - lo,
+                   neighbor_counts=a["counts_nb"][lo:hi])
+        if "neighbors" in a:
+            out["neighbors"] = a["neighbors"][lo:hi].astype(np.int64) - lo
+        return out
+
+    def __del__(self):
+        try:
+            _lib.hipcstone_batched_destroy(self.handle)
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- one-rank domain
 
 

@@ -108,3 +108,27 @@ def gaussian_mixture(n: int, clusters: int = 64, background: float = 0.2, seed: 
         pts[bg] = np.stack([uniform(seed, 20 + d, n)[bg] for d in range(3)], axis=1)
     pts = np.clip(pts, 0.0, np.nextafter(1.0, 0.0))
     return pts[:, 0].astype(dtype), pts[:, 1].astype(dtype), pts[:, 2].astype(dtype)
+
+
+def cloud_mix(num_clouds: int = 512, nmin: int = 1000, nmax: int = 1_000_000, seed: int = 2026,
+              target: float = 64.0, plummer_a: float = 0.1):
+    """The batched workload: `num_clouds` independent fp32 clouds with sizes log-uniform in [nmin, nmax]; each is,
+    50/50, uniform in the unit cube (box [0, 1]^3) or a Plummer sphere of scale radius `plummer_a` truncated at r <= 1
+    (box [-1, 1]^3), with h for about `target` neighbors (uniform_h / plummer_h). Returns the sizes, the per-cloud
+    boxes as (lo, hi) and kinds, and the concatenated x, y, z, h (cloud order)."""
+    u = uniform(seed, 30, num_clouds)
+    sizes = np.exp(np.log(nmin) + (np.log(nmax) - np.log(nmin)) * u).astype(np.int64)
+    kinds = np.where(uniform(seed, 31, num_clouds) < 0.5, "uniform", "plummer")
+    xs, ys, zs, hs, boxes = [], [], [], [], []
+    for c in range(num_clouds):
+        n = int(sizes[c])
+        if kinds[c] == "uniform":
+            x, y, z = uniform_cube(n, seed=seed * 1000 + c, dtype=np.float32)
+            h = np.full(n, uniform_h(n, target), dtype=np.float32)
+            boxes.append((0.0, 1.0))
+        else:
+            x, y, z, r = plummer(n, plummer_a, seed=seed * 1000 + c, dtype=np.float32)
+            h = plummer_h(r, n, plummer_a, target, dtype=np.float32)
+            boxes.append((-1.0, 1.0))
+        xs.append(x), ys.append(y), zs.append(z), hs.append(h)
+    return sizes, boxes, kinds, np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), np.concatenate(hs)

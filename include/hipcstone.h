@@ -39,6 +39,7 @@ typedef enum
 #define HIPCSTONE_FAULT_SORT_LOOKBACK  1u /* a radix sort tile gave up waiting for its predecessor's prefix */
 #define HIPCSTONE_FAULT_WALK_BUDGET    2u /* a neighbor walk used up its step budget (a cycle in child offsets) */
 #define HIPCSTONE_FAULT_MALFORMED_TREE 4u /* a tree walk met a node index outside the tree, or a stack overflow */
+#define HIPCSTONE_FAULT_CAPACITY       8u /* a batched warm step's rebalanced leaf array would exceed its capacity */
 
 /* curve kinds for key encoding (sfc/sfc.hpp:30-40 MortonKey / HilbertKey) */
 enum
@@ -615,6 +616,70 @@ hipcstone_status hipcstone_domain_update_expansion_centers(hipcstone_domain* d, 
 #define HIPCSTONE_DOMAIN_STAGES 8
 hipcstone_status hipcstone_domain_set_timing(hipcstone_domain* d, int enable);
 hipcstone_status hipcstone_domain_stage_times(hipcstone_domain* d, double* ms, int32_t* launches);
+
+/* ---- Batched clouds (no single reference counterpart: the reference runs one cloud per Domain or per pipeline
+ * call, with host synchronizations inside every tree update, tree/csarray_gpu.cu:97-100,200-202,226-228). Up to 512
+ * independent fp32 clouds, stored back to back in cloud order (cloud c holds counts[c] particles, its own box and
+ * bucket size), go through keys -> stable sort + gather -> leaf array -> link -> centers -> layout -> findNeighbors
+ * with every stage over all clouds in one launch. Each cloud's keys (32-bit Morton or Hilbert, shifted into 64-bit
+ * composite keys (c << 54) | (key << 24)), particle order, leaf array and counts (shifted likewise), layout (offset by
+ * the cloud's first particle) and neighbor rows and counts (indices offset likewise) are bit-identical to a plain
+ * pipeline (hipcstone_pipeline_*) run on that cloud alone with the same box, bucket size and ngmax. The linked
+ * nodes are ordered by internal-node slot rather than level by level (children of a node stay consecutive in child
+ * order). hipcstone_batched_build runs the first (full) build; hipcstone_batched_step one warm step: an optional
+ * drift (particle i of cloud c moves as hipcstone_synth_displace_f32 with seed + c moves the cloud's particle
+ * i - first(c); drift_scale 0 skips it), then one updateOctree step of every cloud. A step allocates nothing and never
+ * synchronizes the host, so a caller can capture a sequence of steps in one HIP graph; its leaf arrays have a fixed
+ * capacity (leaf_capacity_factor * particles / smallest bucket size, at least twice the first build's leaf count),
+ * and a step that would exceed it sets HIPCSTONE_FAULT_CAPACITY in the fault word. ---- */
+typedef struct hipcstone_batched hipcstone_batched;
+
+typedef struct
+{
+    uint32_t num_clouds;
+    uint32_t num_waves;
+    size_t num_particles;
+    int32_t num_leaves; /* of the composite leaf array (all clouds, plus one empty leaf per unused cloud slot) */
+    int32_t num_nodes;
+    int32_t leaf_capacity;
+    void* keys; /* composite keys (u64) */
+    void* perm; /* u32, the last sort's permutation (global indices) */
+    void* x;
+    void* y;
+    void* z;
+    void* h;
+    void* leaves; /* num_leaves + 1 composite keys */
+    void* counts; /* u32 */
+    void* layout; /* u32, num_leaves + 1 */
+    void* prefixes;
+    void* child_offsets;
+    void* internal_to_leaf;
+    void* centers;
+    void* sizes;
+    void* cloud_roots; /* int32 node index of every cloud's root */
+    void* neighbors;   /* u32 rows of ngmax per particle (global indices) */
+    void* neighbor_counts;
+} hipcstone_batched_view;
+
+hipcstone_status hipcstone_batched_create(hipcstone_batched** b, int curve, uint32_t num_clouds,
+                                          const uint64_t* counts, const hipcstone_box_f32* boxes,
+                                          const uint32_t* bucket_sizes, uint32_t ngmax, double leaf_capacity_factor);
+hipcstone_status hipcstone_batched_destroy(hipcstone_batched* b);
+/* x, y, z, h of all clouds in cloud order (host or, with device_pointers, device arrays); synchronizes */
+hipcstone_status hipcstone_batched_set_particles(hipcstone_batched* b, hipcstone_stream s, const float* x,
+                                                 const float* y, const float* z, const float* h, int device_pointers);
+hipcstone_status hipcstone_batched_build(hipcstone_batched* b, hipcstone_stream s);
+hipcstone_status hipcstone_batched_step(hipcstone_batched* b, hipcstone_stream s, float drift_scale, uint64_t seed,
+                                        uint64_t step);
+/* synchronizes s (reads the device-side leaf and node counts) */
+hipcstone_status hipcstone_batched_get_view(hipcstone_batched* b, hipcstone_stream s, hipcstone_batched_view* v);
+size_t hipcstone_batched_device_bytes(const hipcstone_batched* b);
+/* Capture `steps` (even) calls of hipcstone_batched_step (step indices first_step .. first_step + steps - 1) on the
+ * created stream s into one executable HIP graph, replacing any earlier one; nothing runs until
+ * hipcstone_batched_replay launches it on s. A replay runs the same drifts again from wherever the data stand. */
+hipcstone_status hipcstone_batched_capture(hipcstone_batched* b, hipcstone_stream s, int32_t steps, float drift_scale,
+                                           uint64_t seed, uint64_t first_step);
+hipcstone_status hipcstone_batched_replay(hipcstone_batched* b, hipcstone_stream s);
 
 /* ---- synthetic workload helper (not an octree stage): x += scale*h*u, u uniform in [-1,1)^3 from a counter-based
  * hash of (seed, step, particle, dimension), clamped into the box; the moving input of multi-step runs ---- */
