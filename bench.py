@@ -589,6 +589,230 @@ def ijloop_cpu_baseline(v, n, m, ngmax, stride=32):
                        f"interaction over bodies [0, {last}) of the same sorted particles and tree: {ms:.0f} ms")}
 
 
+class PlainClouds:
+    """One plain pipeline per cloud (32-bit keys, the cloud's box and bucket size): the per-cloud baseline of the
+    batched workload, stepped the way a batched step moves every cloud (the same drift, cloud by cloud)."""
+
+    def __init__(self, hip, sizes, boxes, buckets, curve, ngmax, x, y, z, h):
+        self.hip = hip
+        self.pipes, self.boxes, self.nb, self.nc = [], [], [], []
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        self.inputs = []
+        for c, n in enumerate(sizes):
+            lo, hi = int(off[c]), int(off[c + 1])
+            self.boxes.append(hip.make_box(boxes[c][0], boxes[c][1], np.float32))
+            self.pipes.append(hip.Pipeline(curve=curve, key_bits=32, real_bits=32, bucket_size=int(buckets[c]),
+                                           ngmax=ngmax))
+            self.inputs.append([hip.DeviceArray.from_numpy(np.ascontiguousarray(a[lo:hi])) for a in (x, y, z, h)])
+            self.nb.append(hip.DeviceArray(int(n) * ngmax, np.uint32))
+            self.nc.append(hip.DeviceArray(int(n), np.uint32))
+
+    def build(self):
+        for c, p in enumerate(self.pipes):
+            d = self.inputs[c]
+            p.sync(*d, d[0].count, self.boxes[c], full_build=True, neighbors=self.nb[c], counts=self.nc[c])
+        self.inputs = None
+
+    def step(self, scale, seed, step):
+        for c, p in enumerate(self.pipes):
+            v = p.view()
+            self.hip.displace(v.x, v.y, v.z, v.h, v.num_particles, scale, seed + c, step, self.boxes[c])
+            p.sync(v.x, v.y, v.z, v.h, v.num_particles, self.boxes[c], full_build=False, neighbors=self.nb[c],
+                   counts=self.nc[c])
+
+
+def batched_bench(args):
+    """The batched workload: 512 fp32 clouds (sizes log-uniform in [1e3, 1e6], uniform or Plummer a = 0.1, h for ~64
+    neighbors), Morton-32 keys, per-cloud bucket sizes from the autotuner, ngmax 128; a step is a drift of 1e-3 h plus
+    keys, sort, one warm update, link, centers, layout and neighbors of every cloud. The batched form runs 20 steps as
+    one captured HIP graph; the baseline runs the same steps as one plain pipeline call per cloud."""
+    import hipcstone as hip
+    from hipcstone import autotune, inputs
+
+    t0 = time.time()
+    sizes, boxes, kinds, x, y, z, h = inputs.cloud_mix(args.clouds)
+    n = int(sizes.sum())
+    tuned = autotune.load()
+    buckets = autotune.cloud_buckets(sizes, kinds, tuned)
+    log(f"input: {args.clouds} clouds, {n} particles ({time.time() - t0:.1f} s); buckets "
+        f"{dict(zip(*np.unique(buckets, return_counts=True)))}")
+    ngmax, drift, G = args.ngmax_batched, 1e-3, args.graph_steps
+    s = hip.Stream()
+    curve = hip.HILBERT if args.batched_curve == "hilbert" else hip.MORTON
+    b = hip.Batched(sizes, [hip.make_box(lo, hi, np.float32) for lo, hi in boxes], buckets, curve=curve,
+                    ngmax=ngmax)
+    b.set_particles(x, y, z, h, stream=s)
+    t1 = time.time()
+    b.build(stream=s)
+    s.sync()
+    log(f"batched build: {time.time() - t1:.2f} s, {b.view(stream=s).num_leaves} leaves")
+    b.capture(G, drift, 99, 0, s)
+    for _ in range(max(1, args.warmup)):
+        b.replay(s)
+    s.sync()
+    K = max(1, args.steps // G)
+    ev = [hip.Event() for _ in range(K + 1)]
+    ev[0].record(s)
+    for k in range(K):
+        b.replay(s)
+        ev[k + 1].record(s)
+    s.sync()
+    faults = hip.device_faults()
+    per_graph = np.array([ev[k].elapsed_ms(ev[k + 1]) for k in range(K)])
+    total_ms = ev[0].elapsed_ms(ev[K])
+    steps = K * G
+    value = n * steps / (total_ms / 1e3) / 1e6
+    v = b.view(stream=s)
+    counts = hip.download_ptr(v.neighbor_counts, n, np.uint32)
+    kbar = float(np.minimum(counts, ngmax).mean())
+    step_bytes = float(sum(pipeline_stage_bytes(n, int(v.num_leaves), int(v.num_nodes), kbar).values()))
+    ms_per_step = total_ms / steps
+    log(f"batched: {ms_per_step:.3f} ms/step, {value:.0f} Mparticles/s, mean neighbors {counts.mean():.1f}")
+    dev_bytes = b.device_bytes()
+    del b
+
+    # the plain per-cloud baseline: the same steps through one pipeline per cloud
+    plain = PlainClouds(hip, sizes, boxes, buckets, curve, ngmax, x, y, z, h)
+    t2 = time.time()
+    plain.build()
+    hip.device_sync()
+    log(f"plain build of {args.clouds} clouds: {time.time() - t2:.2f} s")
+    plain.step(drift, 99, 0)
+    hip.device_sync()
+    P = args.plain_steps
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record()
+    for k in range(P):
+        plain.step(drift, 99, 1 + k)
+    e1.record()
+    hip.device_sync()
+    plain_ms = e0.elapsed_ms(e1) / P
+    plain_value = n / (plain_ms / 1e3) / 1e6
+    log(f"plain per cloud: {plain_ms:.2f} ms/step, {plain_value:.0f} Mparticles/s")
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(1, args.warmup) * G,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"batched: {args.clouds} fp32 clouds, sizes log-uniform in [1e3, 1e6] ({n} particles), uniform "
+                         f"or Plummer a=0.1 50/50, h for ~64 neighbors, {args.batched_curve}-32 keys, autotuned bucket "
+                         f"per cloud, ngmax {ngmax}, {G} drift steps (1e-3 h) captured as one HIP graph"),
+            "clouds": args.clouds, "particles": n, "ngmax": ngmax, "keys": f"{args.batched_curve}-32",
+            "buckets": {str(int(k)): int(c) for k, c in zip(*np.unique(buckets, return_counts=True))},
+            "leaves": int(v.num_leaves), "mean_neighbors": round(float(counts.mean()), 2),
+            "device_bytes": dev_bytes,
+        },
+        "batched_vs_plain": round(value / plain_value, 3),
+        "plain_per_cloud": {"value": round(plain_value, 2), "unit": "Mparticles/s", "ms_per_step": round(plain_ms, 3),
+                            "steps": P, "launches": "one plain pipeline sync (keys..neighbors) per cloud per step"},
+        "latency_ms": {"graph_of_steps_p50": round(float(np.percentile(per_graph, 50)), 3),
+                       "graph_of_steps_p99": round(float(np.percentile(per_graph, 99)), 3)},
+        "pipeline_roofline": {
+            "algorithmic_bytes_per_step": step_bytes,
+            "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
+            "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "device_faults": faults,
+        "autotune": {"source": "hipcstone/tuned.json" if tuned else None,
+                     "bucket": tuned.get("bucket") if tuned else None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def served_bench(args):
+    """The served workload: Poisson arrivals at 200 requests/s (sizes log-uniform in [1e4, 4e6], fp32 Plummer a = 0.05,
+    h for ~100 neighbors, Hilbert-64 keys, bucket 64, ngmax 128), 30 % full builds of a new cloud and 70 % updates of a
+    resident cloud (displacement N(0, (5e-4 h)^2), one warm step), admitted by a 200 GB HBM budget into 4 worker
+    slots (hipcstone_serve_run). Reports per-request latency (arrival to completion) p50/p99 and throughput; then the
+    same requests released all at once through 4 slots and through 1 slot (one request at a time, the plain per-call
+    form), whose throughput ratio is what continuous batching gains."""
+    import hipcstone as hip
+    from hipcstone import inputs
+
+    t0 = time.time()
+    reqs = inputs.served_requests(duration_s=args.serve_seconds, rate=200.0, pool=args.serve_pool)
+    n_total = sum(int(r["n"]) for r in reqs)
+    builds = sum(r["kind"] == 0 for r in reqs)
+    log(f"served: {len(reqs)} requests ({builds} builds), {n_total} particles ({time.time() - t0:.1f} s)")
+
+    def summary(res, st, tag):
+        ok = [r for r in res if r["status"] != hip.SERVE_REJECTED]
+        # from release (the arrival time when requests are released at their arrival times, else the start)
+        lat = np.array([(r["end_s"] - r["arrival_s"]) * 1e3 for r in ok])
+        wait = np.array([(r["admit_s"] - r["arrival_s"]) * 1e3 for r in ok])
+        svc = np.array([(r["end_s"] - r["start_s"]) * 1e3 for r in ok])
+        done = sum(int(q["n"]) for q, r in zip(reqs, res) if r["status"] != hip.SERVE_REJECTED)
+        out = {"requests": len(res), "completed": len(ok), "particles": done,
+               "makespan_s": round(st["makespan_s"], 3),
+               "throughput_Mparticles_s": round(done / st["makespan_s"] / 1e6, 2),
+               "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                              "max": round(float(lat.max()), 3)},
+               "queue_ms": {"p50": round(float(np.percentile(wait, 50)), 3), "p99": round(float(np.percentile(wait, 99)), 3)},
+               "service_ms": {"p50": round(float(np.percentile(svc, 50)), 3), "p99": round(float(np.percentile(svc, 99)), 3)},
+               "evictions": st["evictions"], "rebuilt": sum(r["status"] == hip.SERVE_REBUILT for r in res),
+               "rejected": sum(r["status"] == hip.SERVE_REJECTED for r in res),
+               "peak_resident_GB": round(max(r["resident_bytes"] for r in res) / 1e9, 2)}
+        log(f"{tag}: {out['throughput_Mparticles_s']} Mparticles/s, latency p50 {out['latency_ms']['p50']} ms, "
+            f"p99 {out['latency_ms']['p99']} ms")
+        return out
+
+    res, st = hip.serve_run(reqs, hbm_budget=200 * 10**9, slots=args.serve_slots, realtime=True)
+    live = summary(res, st, "served at 200 req/s")
+    res, st = hip.serve_run(reqs, hbm_budget=200 * 10**9, slots=args.serve_slots, realtime=False)
+    sat = summary(res, st, f"saturated, {args.serve_slots} slots")
+    res, st = hip.serve_run(reqs, hbm_budget=200 * 10**9, slots=1, realtime=False)
+    one = summary(res, st, "saturated, 1 slot")
+    result = {
+        "metric": METRIC,
+        "value": live["throughput_Mparticles_s"],
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 0,
+        "ms_per_step": round(live["makespan_s"] * 1e3 / max(1, len(reqs)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"served: Poisson arrivals at 200 req/s for {args.serve_seconds} s, sizes log-uniform in "
+                         "[1e4, 4e6], fp32 Plummer a=0.05, Hilbert-64, bucket 64, ngmax 128, 30% builds / 70% updates "
+                         f"(N(0,(5e-4 h)^2)), 200 GB HBM admission, {args.serve_slots} worker slots"),
+            "requests": len(reqs), "builds": int(builds), "particles": n_total},
+        "latency_ms": live["latency_ms"],
+        "served": live,
+        "saturated": sat,
+        "saturated_one_slot": one,
+        "slots_vs_one_slot": round(sat["throughput_Mparticles_s"] / one["throughput_Mparticles_s"], 3),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def autotune_run(args):
+    import hipcstone as hip
+    from hipcstone import autotune
+
+    curve = hip.HILBERT if args.batched_curve == "hilbert" else hip.MORTON
+    tuned = autotune.tune(hip, clouds_per_class=args.tune_clouds, curve=curve, log=log)
+    tuned["curve"] = args.batched_curve
+    out = args.tune_out or os.path.join(ROOT, "gpurun_out", "tuned.json")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    autotune.save(tuned, out)
+    print(json.dumps(tuned), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -598,11 +822,26 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "c1", "halo", "domain", "grav", "ijloop"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "c1", "halo", "domain", "grav", "ijloop", "batched", "autotune",
+                                               "served"],
+                    default="plain",
                     help="plain: the 16.8M neighbor pipeline (default); c1: the 1M fp64 full-build pipeline; "
                          "halo: the 134M rank-share chain; "
                          "domain: one-rank Domain::sync + neighbors at 16.8M; grav: one-rank Domain::syncGrav at "
-                         "16.8M; ijloop: the ijLoop example at 16.8M")
+                         "16.8M; ijloop: the ijLoop example at 16.8M; batched: 512 clouds in one graph vs one "
+                         "plain pipeline per cloud; autotune: the bucket-size sweep of the batched clouds")
+    ap.add_argument("--clouds", type=int, default=512)
+    ap.add_argument("--ngmax-batched", type=int, default=128)
+    ap.add_argument("--graph-steps", type=int, default=20)
+    ap.add_argument("--batched-curve", choices=["hilbert", "morton"], default="hilbert",
+                    help="32-bit key curve of the batched clouds (node boxes are decoded as Hilbert keys, as the "
+                         "reference's nodeFpCenters does, so Morton trees give truncated neighbor sets there and here)")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--tune-clouds", type=int, default=8)
+    ap.add_argument("--tune-out", default=None)
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-pool", type=int, default=6)
     ap.add_argument("--n-halo", type=int, default=134217728)
     ap.add_argument("--halo-cpu-sample", type=int, default=8388608,
                     help="particles of the CPU baseline sample of the halo workload")
@@ -645,6 +884,15 @@ def main():
         return
     if args.workload == "grav":
         grav_bench(args)
+        return
+    if args.workload == "batched":
+        batched_bench(args)
+        return
+    if args.workload == "autotune":
+        autotune_run(args)
+        return
+    if args.workload == "served":
+        served_bench(args)
         return
 
     import hipcstone as hip

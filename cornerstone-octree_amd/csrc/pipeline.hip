@@ -36,6 +36,7 @@ struct PipelineBase
     virtual hipcstone_status halos(hipcstone_stream s, int32_t numRanks, int32_t rank, float ext, void* xo, void* yo,
                                    void* zo, void* ho, size_t capacity, size_t* numLocal) = 0;
     virtual void view(hipcstone_pipeline_view* v) const = 0;
+    virtual size_t deviceBytes() const = 0;
 };
 
 // curve-dispatched wrappers around the per-type C ABI
@@ -377,6 +378,25 @@ public:
         v->local_layout    = localLayout_.ptr;
     }
 
+    // device memory held by the pipeline's buffers
+    size_t deviceBytes() const override
+    {
+        size_t b = 0;
+        for (const DeviceBuf* d : {&keys_, &keyBuf_, &perm_, &valBuf_, &sortWs_, &segWs_, &updWs_, &splitKeys_,
+                                   &splitCounts_, &splitCounters_, &splitSortWs_, &prefixes_, &childOffsets_,
+                                   &parents_, &levelRange_, &internalToLeaf_, &leafToInternal_, &buildWs_, &centers_,
+                                   &sizes_, &layout_, &scanWs_, &assignOff_, &assignBnd_, &assignCnt_, &assignWs_,
+                                   &flags_, &searchC_, &searchS_, &localLayout_, &haloWs_})
+            b += d->bytes;
+        for (int t = 0; t < 2; ++t)
+        {
+            b += tree_[t].bytes + counts_[t].bytes + frontier_[t].bytes;
+            for (int a = 0; a < 4; ++a)
+                b += xyzh_[t][a].bytes;
+        }
+        return b;
+    }
+
 private:
     // warm-step stable sort + gather through the buckets of the previous leaf array (numLeaves_ > 0); the radix
     // sort when a bucket is too large
@@ -641,6 +661,8 @@ hipcstone_status hipcstone_pipeline_destroy(hipcstone_pipeline* p)
     delete p;
     return HIPCSTONE_OK;
 }
+
+size_t hipcstone_pipeline_device_bytes(const hipcstone_pipeline* p) { return p ? p->impl->deviceBytes() : 0; }
 
 hipcstone_status hipcstone_pipeline_sync(hipcstone_pipeline* p, hipcstone_stream s, const void* x, const void* y,
                                          const void* z, const void* h, size_t n, const void* box, int full_build,

@@ -21,6 +21,16 @@ __global__ __launch_bounds__(256) void displaceKernel(T* __restrict__ x, T* __re
     displaceAt(x, y, z, h, i, uint64_t(i), scale, seed, step, box);
 }
 
+template<class T>
+__global__ __launch_bounds__(256) void displaceNormalKernel(T* __restrict__ x, T* __restrict__ y, T* __restrict__ z,
+                                                            const T* __restrict__ h, size_t n, T sigma, uint64_t seed,
+                                                            uint64_t step, typename BoxOf<T>::type box)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    displaceNormalAt(x, y, z, h, i, uint64_t(i), sigma, seed, step, box);
+}
+
 // value k of stream `stream` under `seed`, in [0, 1): the counter-based splitmix64 sequence of the host-side input
 // generators (same bits for the same (seed, stream, k)).
 __device__ inline double uniformAt(uint64_t seed, uint64_t stream, uint64_t k)
@@ -146,6 +156,17 @@ hipcstone_status hipcstone_synth_displace_f64(hipcstone_stream s, double* x, dou
                                               const hipcstone_box_f64* box)
 {
     return displace<double>(s, x, y, z, h, n, scale, seed, step, box);
+}
+
+hipcstone_status hipcstone_synth_displace_normal_f32(hipcstone_stream s, float* x, float* y, float* z,
+                                                     const float* h, size_t n, float sigma, uint64_t seed,
+                                                     uint64_t step, const hipcstone_box_f32* box)
+{
+    if (n == 0) return HIPCSTONE_OK;
+    if (!x || !y || !z || !h || !box) return invalid("synth_displace_normal: null pointer");
+    displaceNormalKernel<float><<<ceilDiv(n, 256), 256, 0, stream(s)>>>(x, y, z, h, n, sigma, seed, step, *box);
+    HC_CHECK_LAUNCH("displaceNormalKernel");
+    return HIPCSTONE_OK;
 }
 
 hipcstone_status hipcstone_synth_gaussian_mixture_f32(hipcstone_stream s, float* x, float* y, float* z, float* h,

@@ -899,3 +899,52 @@ class Domain:
             _lib.hipcstone_domain_destroy(self.handle)
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- served clouds
+
+SERVE_BUILD, SERVE_UPDATE = 0, 1
+SERVE_BUILT, SERVE_UPDATED, SERVE_REBUILT, SERVE_REJECTED = 0, 1, 2, 3
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("cloud", C.c_uint32), ("kind", C.c_uint32), ("n", C.c_uint64),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("h", C.c_void_p),
+                ("arrival_s", C.c_double), ("seed", C.c_uint64)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_uint32), ("slot", C.c_uint32), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("start_s", C.c_double), ("end_s", C.c_double), ("digest", C.c_uint64),
+                ("num_leaves", C.c_int32), ("pad", C.c_uint32), ("resident_bytes", C.c_uint64)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_int32), ("bucket_size", C.c_uint32), ("ngmax", C.c_uint32),
+                ("curve", C.c_int32), ("sigma", C.c_float), ("box_lo", C.c_float), ("box_hi", C.c_float),
+                ("realtime", C.c_int32), ("digests", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("makespan_s", C.c_double), ("evictions", C.c_uint64)]
+
+
+def serve_run(requests, hbm_budget=200 << 30, slots=4, bucket_size=64, ngmax=128, curve=HILBERT, sigma=5e-4,
+              box=(-1.0, 1.0), realtime=True, digests=False):
+    """Runs requests (dicts: id, cloud, kind, n, x, y, z, h (float32 host arrays, kept alive by the caller),
+    arrival_s, seed) through hipcstone_serve_run; returns (list of result dicts, stats dict)."""
+    arr = (ServeRequest * max(1, len(requests)))()
+    for i, r in enumerate(requests):
+        q = arr[i]
+        q.id, q.cloud, q.kind, q.n = int(r["id"]), int(r["cloud"]), int(r["kind"]), int(r["n"])
+        for k in "xyzh":
+            a = r.get(k)
+            setattr(q, k, a.ctypes.data if a is not None else None)
+        q.arrival_s, q.seed = float(r["arrival_s"]), int(r.get("seed", 0))
+    res = (ServeResult * max(1, len(requests)))()
+    cfg = ServeConfig(int(hbm_budget), int(slots), int(bucket_size), int(ngmax), int(curve), float(sigma),
+                      float(box[0]), float(box[1]), int(bool(realtime)), int(bool(digests)))
+    stats = ServeStats()
+    _check(_lib.hipcstone_serve_run(C.byref(cfg), arr, C.c_size_t(len(requests)), res, C.byref(stats)), "serve_run")
+    out = [{f: getattr(res[i], f) for f, _ in ServeResult._fields_ if f != "pad"} for i in range(len(requests))]
+    return out, {"makespan_s": stats.makespan_s, "evictions": int(stats.evictions)}

@@ -132,3 +132,38 @@ def cloud_mix(num_clouds: int = 512, nmin: int = 1000, nmax: int = 1_000_000, se
             boxes.append((-1.0, 1.0))
         xs.append(x), ys.append(y), zs.append(z), hs.append(h)
     return sizes, boxes, kinds, np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), np.concatenate(hs)
+
+
+def served_requests(duration_s: float = 5.0, rate: float = 200.0, seed: int = 5150, nmin: int = 10_000,
+                    nmax: int = 4_000_000, build_frac: float = 0.3, a: float = 0.05, target: float = 100.0,
+                    pool: int = 6):
+    """The served workload: Poisson arrivals at `rate` requests/s over `duration_s`; sizes log-uniform in
+    [nmin, nmax]; a fraction `build_frac` are full builds of a new cloud (fp32 Plummer a, truncated at r <= 1, h for
+    `target` neighbors), the rest updates of an earlier cloud (chosen uniformly among the clouds built so far). A
+    build of n particles takes the first n of one of `pool` Plummer samples of nmax particles (i.i.d. draws, so any
+    prefix is itself a Plummer sample) with its own h for n. Returns the request dicts (host arrays kept in them)."""
+    gap = -np.log(1.0 - uniform(seed, 50, 1 << 16)) / rate
+    arrivals = np.cumsum(gap)
+    arrivals = arrivals[arrivals < duration_s]
+    m = arrivals.size
+    kind_u = uniform(seed, 51, m)
+    size_u = uniform(seed, 52, m)
+    pick_u = uniform(seed, 53, m)
+    seeds = (uniform(seed, 54, m) * 2**52).astype(np.uint64)
+    samples = [plummer(nmax, a, seed=seed * 31 + p, dtype=np.float32) for p in range(pool)]
+    reqs, clouds = [], []
+    for i in range(m):
+        build = kind_u[i] < build_frac or not clouds
+        if build:
+            n = int(np.exp(np.log(nmin) + (np.log(nmax) - np.log(nmin)) * size_u[i]))
+            x, y, z, r = samples[len(clouds) % pool]
+            h = plummer_h(r[:n], n, a, target, dtype=np.float32)
+            cloud = {"id": len(clouds), "n": n, "x": x[:n], "y": y[:n], "z": z[:n], "h": h}
+            clouds.append(cloud)
+            reqs.append({"id": i, "cloud": cloud["id"], "kind": 0, "n": n, "x": cloud["x"], "y": cloud["y"],
+                         "z": cloud["z"], "h": h, "arrival_s": float(arrivals[i]), "seed": int(seeds[i])})
+        else:
+            cloud = clouds[min(int(pick_u[i] * len(clouds)), len(clouds) - 1)]
+            reqs.append({"id": i, "cloud": cloud["id"], "kind": 1, "n": cloud["n"], "x": None, "y": None, "z": None,
+                         "h": None, "arrival_s": float(arrivals[i]), "seed": int(seeds[i])})
+    return reqs

@@ -514,6 +514,8 @@ typedef struct
 hipcstone_status hipcstone_pipeline_create(hipcstone_pipeline** p, int curve, int key_bits, int real_bits,
                                            uint32_t bucket_size, uint32_t ngmax);
 hipcstone_status hipcstone_pipeline_destroy(hipcstone_pipeline* p);
+/* device memory the pipeline holds (its buffers grow on demand and are kept) */
+size_t hipcstone_pipeline_device_bytes(const hipcstone_pipeline* p);
 hipcstone_status hipcstone_pipeline_sync(hipcstone_pipeline* p, hipcstone_stream s, const void* x, const void* y,
                                          const void* z, const void* h, size_t n, const void* box, int full_build,
                                          uint32_t* neighbors, uint32_t* neighbors_count);
@@ -681,11 +683,96 @@ hipcstone_status hipcstone_batched_capture(hipcstone_batched* b, hipcstone_strea
                                            uint64_t seed, uint64_t first_step);
 hipcstone_status hipcstone_batched_replay(hipcstone_batched* b, hipcstone_stream s);
 
+/* ---- Served clouds (no reference counterpart: the reference has no request loop). hipcstone_serve_run runs a
+ * list of requests through a loop with continuous batching: requests are released at their arrival times (realtime)
+ * or all at once into one FIFO; a request is admitted when one of `slots` workers is free, its cloud has no running
+ * or earlier queued request (each cloud's requests run in order), and, for a build, the device memory it is modelled
+ * to need fits hbm_budget next to the resident clouds after evicting least-recently-used idle clouds. Every worker
+ * runs its requests on its own HIP stream, so admitted requests overlap on the device. A build (kind
+ * HIPCSTONE_SERVE_BUILD) uploads the cloud's n particles (host x, y, z, h; a resident cloud of the same id is
+ * replaced) and runs a full pipeline sync (Hilbert or Morton 64-bit keys per `curve`, bucket_size, ngmax, box
+ * [box_lo, box_hi]^3); the cloud's pipeline then stays resident. An update (HIPCSTONE_SERVE_UPDATE) displaces the
+ * resident cloud's particles by N(0, (sigma h)^2) (hipcstone_synth_displace_normal_f32 with the request's seed and
+ * id) and runs one warm step from the cloud's previous tree; an update of an evicted cloud first rebuilds it from the
+ * request's x, y, z, h (the cloud's build particles). Results hold arrival, admission, start and end times (seconds
+ * from the start of the run), the worker slot, the status, the cloud's leaf count and, with digests on, an
+ * order-independent 64-bit digest of the request's keys, neighbor counts and stored neighbor rows. ---- */
+enum
+{
+    HIPCSTONE_SERVE_BUILD  = 0,
+    HIPCSTONE_SERVE_UPDATE = 1
+};
+enum
+{
+    HIPCSTONE_SERVE_BUILT    = 0,
+    HIPCSTONE_SERVE_UPDATED  = 1,
+    HIPCSTONE_SERVE_REBUILT  = 2,
+    HIPCSTONE_SERVE_REJECTED = 3
+};
+
+typedef struct
+{
+    uint64_t id;
+    uint32_t cloud;
+    uint32_t kind;
+    uint64_t n;
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* h;
+    double arrival_s;
+    uint64_t seed;
+} hipcstone_serve_request;
+
+typedef struct
+{
+    uint64_t id;
+    uint32_t status;
+    uint32_t slot;
+    double arrival_s;
+    double admit_s;
+    double start_s;
+    double end_s;
+    uint64_t digest;
+    int32_t num_leaves;
+    uint32_t pad;
+    uint64_t resident_bytes; /* device bytes of all resident clouds after this request */
+} hipcstone_serve_result;
+
+typedef struct
+{
+    uint64_t hbm_budget;
+    int32_t slots;
+    uint32_t bucket_size;
+    uint32_t ngmax;
+    int32_t curve;
+    float sigma;
+    float box_lo;
+    float box_hi;
+    int32_t realtime;
+    int32_t digests;
+} hipcstone_serve_config;
+
+typedef struct
+{
+    double makespan_s;
+    uint64_t evictions;
+} hipcstone_serve_stats;
+
+hipcstone_status hipcstone_serve_run(const hipcstone_serve_config* cfg, const hipcstone_serve_request* requests,
+                                     size_t num_requests, hipcstone_serve_result* results,
+                                     hipcstone_serve_stats* stats);
+
 /* ---- synthetic workload helper (not an octree stage): x += scale*h*u, u uniform in [-1,1)^3 from a counter-based
  * hash of (seed, step, particle, dimension), clamped into the box; the moving input of multi-step runs ---- */
 hipcstone_status hipcstone_synth_displace_f32(hipcstone_stream s, float* x, float* y, float* z, const float* h,
                                               size_t n, float scale, uint64_t seed, uint64_t step,
                                               const hipcstone_box_f32* box);
+/* the same with a normal displacement: x_i += sigma * h_i * g_i, g_i ~ N(0, 1)^3 (Box-Muller from the counter-based
+ * uniforms of (seed, step, i)), clamped into the box */
+hipcstone_status hipcstone_synth_displace_normal_f32(hipcstone_stream s, float* x, float* y, float* z,
+                                                     const float* h, size_t n, float sigma, uint64_t seed,
+                                                     uint64_t step, const hipcstone_box_f32* box);
 hipcstone_status hipcstone_synth_displace_f64(hipcstone_stream s, double* x, double* y, double* z, const double* h,
                                               size_t n, double scale, uint64_t seed, uint64_t step,
                                               const hipcstone_box_f64* box);
