@@ -726,7 +726,8 @@ def batched_bench(args):
                      "bucket": tuned.get("bucket") if tuned else None},
         "cpu_baseline": None,
     }
-    print(json.dumps(result), flush=True)
+    del plain
+    return result
 
 
 def served_bench(args):
@@ -797,7 +798,7 @@ def served_bench(args):
         "slots_vs_one_slot": round(sat["throughput_Mparticles_s"] / one["throughput_Mparticles_s"], 3),
         "cpu_baseline": None,
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def autotune_run(args):
@@ -822,6 +823,8 @@ def main():
     ap.add_argument("--ngmax", type=int, default=150)
     ap.add_argument("--cpu-stride", type=int, default=32, help="neighbor sample stride of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="plain workload only: skip the batched and served summaries added to its line")
     ap.add_argument("--workload", choices=["plain", "c1", "halo", "domain", "grav", "ijloop", "batched", "autotune",
                                                "served"],
                     default="plain",
@@ -886,13 +889,13 @@ def main():
         grav_bench(args)
         return
     if args.workload == "batched":
-        batched_bench(args)
+        print(json.dumps(batched_bench(args)), flush=True)
         return
     if args.workload == "autotune":
         autotune_run(args)
         return
     if args.workload == "served":
-        served_bench(args)
+        print(json.dumps(served_bench(args)), flush=True)
         return
 
     import hipcstone as hip
@@ -1007,7 +1010,40 @@ def main():
         result["cpu_baseline"] = cb
     else:
         result["cpu_baseline"] = None
+    if not args.no_extras:
+        # the other two forms of the metric, run after the timed region of the plain pipeline (their own timed
+        # regions): the 512-cloud batched mix against one plain pipeline per cloud, and the served request loop
+        del nb, nc, d, pipe
+        result["batched"] = extra_summary(batched_bench, args, ("value", "unit", "ms_per_step", "batched_vs_plain",
+                                                                "plain_per_cloud", "pipeline_roofline", "steps"),
+                                          lambda r: {"clouds": r["config"]["clouds"], "particles": r["config"]["particles"],
+                                                     "keys": r["config"]["keys"], "buckets": r["config"]["buckets"],
+                                                     "mean_neighbors": r["config"]["mean_neighbors"]})
+        result["served"] = extra_summary(served_bench, args, ("value", "unit", "latency_ms", "slots_vs_one_slot"),
+                                         lambda r: {"requests": r["config"]["requests"],
+                                                    "builds": r["config"]["builds"],
+                                                    "queue_ms": r["served"]["queue_ms"],
+                                                    "peak_resident_GB": r["served"]["peak_resident_GB"],
+                                                    "saturated_Mparticles_s": r["saturated"]["throughput_Mparticles_s"],
+                                                    "saturated_one_slot_Mparticles_s":
+                                                        r["saturated_one_slot"]["throughput_Mparticles_s"]})
     print(json.dumps(result), flush=True)
+
+
+def extra_summary(fn, args, keys, more):
+    """Runs one of the other workloads and keeps its headline fields; a failure is reported in the line instead of
+    losing the plain pipeline's result."""
+    try:
+        t0 = time.time()
+        r = fn(args)
+        out = {k: r[k] for k in keys if k in r}
+        out.update(more(r))
+        out["workload"] = r["config"]["workload"]
+        log(f"{fn.__name__}: {time.time() - t0:.1f} s")
+        return out
+    except Exception as e:  # noqa: BLE001 - reported, not hidden
+        log(f"{fn.__name__} failed: {e!r}")
+        return {"error": repr(e)}
 
 
 if __name__ == "__main__":
