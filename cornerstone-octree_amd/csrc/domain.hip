@@ -515,21 +515,7 @@ public:
     } while (0)
         HC_STAGE(0, updateBox(s, x, y, z));
         HC_STAGE(1, A::keys(s, x, y, z, keys_.as<K>(), n, &box_));
-        bool bucketed = false;
-        if (numGlobal_ > 0)
-        {
-            // later calls: through the buckets of the previous global leaf array (same result as the radix sort)
-            HC_STAGE(2, (sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, gTree_[gCur_].template as<K>(), numGlobal_, x, y,
-                                                 z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(), xyzh_[2].as<T>(),
-                                                 xyzh_[3].as<T>(), perm_.as<uint32_t>(), keyBuf_.as<K>(),
-                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_, &bucketed)));
-        }
-        if (!bucketed)
-        {
-            HC_STAGE(2, A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xyzh_[0].as<T>(),
-                                      xyzh_[1].as<T>(), xyzh_[2].as<T>(), xyzh_[3].as<T>(), keyBuf_.as<K>(),
-                                      valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes));
-        }
+        HC_STAGE(2, sortAndGather(s, x, y, z, h, n));
         HC_STAGE(3, globalUpdate(s));
         if (first_)
         {
@@ -711,6 +697,26 @@ private:
         }
         A::makeBox(lo_, hi_, bnd_, &box_);
         return HIPCSTONE_OK;
+    }
+
+    // stable sort + gather of the call's particles into xyzh_ (one timed stage): later calls go through the buckets
+    // of the previous global leaf array, which hands a step with a bucket above its limit to the radix sort (the
+    // same result either way)
+    hipcstone_status sortAndGather(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, size_t n)
+    {
+        bool bucketed = false;
+        if (numGlobal_ > 0)
+        {
+            if (auto e = sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, gTree_[gCur_].template as<K>(), numGlobal_, x, y,
+                                                 z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(), xyzh_[2].as<T>(),
+                                                 xyzh_[3].as<T>(), perm_.as<uint32_t>(), keyBuf_.as<K>(),
+                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_, &bucketed))
+                return e;
+        }
+        if (bucketed) return HIPCSTONE_OK;
+        return A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(),
+                             xyzh_[2].as<T>(), xyzh_[3].as<T>(), keyBuf_.as<K>(), valBuf_.as<uint32_t>(),
+                             sortWs_.ptr, sortWs_.bytes);
     }
 
     hipcstone_status reserveParticles(size_t n)

@@ -170,3 +170,30 @@ def test_domain_sync_grav_steps(hip, oracle, kd, T, bucket, bucket_focus, theta)
     dom.update_expansion_centers(d[0], d[1], d[2], d[4])
     ref_c, _ = ref.expansion_centers(o["x"], o["y"], o["z"], ms, theta)
     assert np.array_equal(dom.expansion_centers().view(np.uint8), ref_c.view(np.uint8))
+
+
+def test_domain_sync_crowded_bucket_takes_the_radix_sort(hip, oracle):
+    """A later sync whose input crowds more than 2048 particles into one leaf of the previous global leaf array hands
+    that step's sort from the buckets to the radix sort; the result equals the oracle's domain either way."""
+    n, T, kd = 60000, np.float32, np.uint64
+    x, y, z, h = plummer_input(n, T, seed=37)
+    dom = hip.Domain(hip.make_box(-1.0, 1.0, T), key_bits=64, bucket_size=64, bucket_size_focus=16)
+    ref = CpuDomain(oracle, kd, T, 64, 16, -1.0, 1.0)
+    d = [dev(hip, a) for a in (x, y, z, h)]
+    for step in range(3):
+        print(f"domain crowded bucket: step {step}", flush=True)
+        xs, ys, zs, hs = (a.download() for a in d)
+        o = ref.sync(xs, ys, zs, hs)
+        dom.sync(*d, n)
+        g = dom.arrays()
+        assert np.array_equal(g["keys"], o["keys"])
+        for name, a in zip("xyzh", d):
+            assert np.array_equal(a.download(), o[name]), name
+        assert np.array_equal(g["global_leaves"], o["gtree"]) and np.array_equal(g["leaves"], o["fleaves"])
+        nxt = [np.array(o[k], copy=True) for k in "xyzh"]
+        if step == 0:
+            # 3000 particles of one region onto one point inside the box
+            sel = np.arange(n // 2, n // 2 + 3000)
+            nxt[0][sel], nxt[1][sel], nxt[2][sel] = T(0.013), T(-0.021), T(0.007)
+        for a, m in zip(d, nxt):
+            a.upload(np.ascontiguousarray(m))
