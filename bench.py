@@ -806,8 +806,12 @@ def autotune_run(args):
     from hipcstone import autotune
 
     curve = hip.HILBERT if args.batched_curve == "hilbert" else hip.MORTON
+    waves, wtimes = autotune.tune_neighbor_waves(hip, log=log)
+    hip.set_neighbor_waves(waves)
     tuned = autotune.tune(hip, clouds_per_class=args.tune_clouds, curve=curve, log=log)
     tuned["curve"] = args.batched_curve
+    tuned["neighbor_waves"] = int(waves)
+    tuned["neighbor_waves_ms"] = {str(k): round(v, 4) for k, v in wtimes.items()}
     out = args.tune_out or os.path.join(ROOT, "gpurun_out", "tuned.json")
     os.makedirs(os.path.dirname(out), exist_ok=True)
     autotune.save(tuned, out)
@@ -873,6 +877,13 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "of the pipeline is a design in DESIGN.md, not a runnable path", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "autotune":
+        import hipcstone as hip
+        from hipcstone import autotune
+
+        waves = autotune.apply(hip)
+        log(f"neighbor search: {waves} waves per workgroup")
 
     if args.workload == "halo":
         halo_bench(args)
@@ -978,6 +989,7 @@ def main():
                          "displacement, findNeighbors ngmax 150"),
             "particles": n, "bucket_size": 64, "ngmax": ngmax, "keys": "hilbert-64",
             "leaves": int(v.num_leaves), "nodes": int(v.num_nodes), "mean_neighbors": round(float(counts.mean()), 2),
+            "neighbor_waves_per_workgroup": hip.get_neighbor_waves(),
         },
         "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
                        "p99": round(float(np.percentile(per_step, 99)), 3),

@@ -30,8 +30,9 @@
 namespace hipcstone
 {
 
-constexpr int kNbThreads = 256;
-constexpr int kNbWaves   = kNbThreads / kWave;
+// waves per workgroup: 4 by default; 2 and 8 are the other shapes the autotuner measures
+// (hipcstone_set_neighbor_waves). Every wave owns its own targets, stack and leaf list, so the shape changes only
+// how waves are grouped onto CUs (LDS per workgroup, scheduling granularity), never a result.
 constexpr int kMaxDepth  = 22; // 21 levels below the root for 64-bit keys
 constexpr int kLeafList  = 64;
 
@@ -85,7 +86,7 @@ struct alignas(4 * sizeof(T)) TargetRec
     uint32_t pad;
 };
 
-template<class T>
+template<class T, int kNbWaves>
 struct NbShared
 {
     uint64_t mask[kNbWaves][kMaxDepth][8];
@@ -106,8 +107,8 @@ struct NbShared
 // (waves[3 g], waves[3 g + 1], cloud = waves[3 g + 2], root = roots[cloud]); rows and counts are indexed by the
 // particle index itself (firstId = 0), and a child block may lie before its parent (the batched link places children
 // by internal-node slot, not level by level), so only the bounds and depth guards and the step budget apply.
-template<class T, bool AnyPbc, bool SlotMajor, bool Segmented = false>
-__global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
+template<class T, bool AnyPbc, bool SlotMajor, bool Segmented, int kNbWaves>
+__global__ __launch_bounds__(kNbWaves * kWave) void findNeighborsKernel(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, const T* __restrict__ h,
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
     const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(kNbThreads) void findNeighborsKernel(
     uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded, uint32_t* __restrict__ faults,
     const uint32_t* __restrict__ waves = nullptr, uint32_t numWaves = 0, const int32_t* __restrict__ roots = nullptr)
 {
-    __shared__ NbShared<T> sh;
+    __shared__ NbShared<T, kNbWaves> sh;
     const int lane       = laneId();
     const int w          = threadIdx.x / kWave;
     const uint32_t block = xcdContiguousBlock(blockIdx.x, numBlocksPadded);
@@ -526,11 +527,20 @@ hipcstone_status findNeighborsSegmented(hipStream_t s, const T* x, const T* y, c
     if (ngmax > (1u << 20)) return invalid("findNeighbors: ngmax above 2^20");
     uint32_t* faults = faultWord();
     if (!faults) return HIPCSTONE_HIP_ERROR;
-    unsigned blocks = ceilDiv(size_t(numWaves), kNbWaves);
-    blocks          = (blocks + kXcds - 1) / kXcds * kXcds;
-    findNeighborsKernel<T, false, false, true><<<blocks, kNbThreads, 0, s>>>(
-        x, y, z, h, 0u, 0u, box, childOffsets, internalToLeaf, layout, centers, sizes, numNodesCap, 1.0f, ngmax,
-        neighbors, neighborsCount, blocks, faults, waves, numWaves, roots);
+    auto launch = [&](auto wv) {
+        constexpr int W = decltype(wv)::value;
+        unsigned blocks = ceilDiv(size_t(numWaves), W);
+        blocks          = (blocks + kXcds - 1) / kXcds * kXcds;
+        findNeighborsKernel<T, false, false, true, W><<<blocks, W * kWave, 0, s>>>(
+            x, y, z, h, 0u, 0u, box, childOffsets, internalToLeaf, layout, centers, sizes, numNodesCap, 1.0f, ngmax,
+            neighbors, neighborsCount, blocks, faults, waves, numWaves, roots);
+    };
+    switch (neighborWaves())
+    {
+    case 2: launch(std::integral_constant<int, 2>{}); break;
+    case 8: launch(std::integral_constant<int, 8>{}); break;
+    default: launch(std::integral_constant<int, 4>{}); break;
+    }
     HC_CHECK_LAUNCH("findNeighborsKernel (segmented)");
     return HIPCSTONE_OK;
 }
@@ -557,17 +567,26 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
     if (ngmax > (1u << 20)) return invalid("findNeighbors: ngmax above 2^20");
     if (numNodes <= 0) return invalid("findNeighbors: numNodes must be positive");
     size_t groups   = (size_t(lastId - firstId) + kWave - 1) / kWave;
-    size_t threads  = groups * kWave;
-    unsigned blocks = ceilDiv(threads, kNbThreads);
-    blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
     uint32_t* faults = faultWord();
     if (!faults) return HIPCSTONE_HIP_ERROR;
     bool anyPbc     = box->bnd[0] == HIPCSTONE_PERIODIC || box->bnd[1] == HIPCSTONE_PERIODIC ||
                   box->bnd[2] == HIPCSTONE_PERIODIC;
+    auto shaped = [&](auto pbc, auto slots, auto wv) {
+        constexpr int W = decltype(wv)::value;
+        unsigned blocks = ceilDiv(groups, W);
+        blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
+        findNeighborsKernel<T, decltype(pbc)::value, decltype(slots)::value, false, W>
+            <<<blocks, W * kWave, 0, stream(s)>>>(x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf,
+                                                  layout, centers, sizes, numNodes, ext, ngmax, neighbors,
+                                                  neighborsCount, blocks, faults);
+    };
     auto launch = [&](auto pbc, auto slots) {
-        findNeighborsKernel<T, decltype(pbc)::value, decltype(slots)::value><<<blocks, kNbThreads, 0, stream(s)>>>(
-            x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext,
-            ngmax, neighbors, neighborsCount, blocks, faults);
+        switch (neighborWaves())
+        {
+        case 2: shaped(pbc, slots, std::integral_constant<int, 2>{}); break;
+        case 8: shaped(pbc, slots, std::integral_constant<int, 8>{}); break;
+        default: shaped(pbc, slots, std::integral_constant<int, 4>{}); break;
+        }
     };
     if (anyPbc && slotMajor) launch(std::true_type{}, std::true_type{});
     else if (anyPbc) launch(std::true_type{}, std::false_type{});

@@ -70,6 +70,10 @@ hipcstone_status checkFaults(uint32_t faults)
     return HIPCSTONE_DEVICE_FAULT;
 }
 
+static std::atomic<int> g_neighborWaves{4};
+
+int neighborWaves() { return g_neighborWaves.load(); }
+
 static std::atomic<uint32_t> g_lookbackLimit{0};
 
 uint32_t sortLookbackLimit()
@@ -139,6 +143,16 @@ hipcstone_status hipcstone_device_faults(hipcstone_stream s, uint32_t* faults)
     if (faults) *faults = v;
     return checkFaults(v);
 }
+
+hipcstone_status hipcstone_set_neighbor_waves(int32_t waves)
+{
+    if (waves == 0) waves = 4;
+    if (waves != 2 && waves != 4 && waves != 8) return invalid("set_neighbor_waves: 2, 4 or 8 waves per workgroup");
+    g_neighborWaves.store(waves);
+    return HIPCSTONE_OK;
+}
+
+int32_t hipcstone_get_neighbor_waves(void) { return g_neighborWaves.load(); }
 
 hipcstone_status hipcstone_set_sort_lookback_limit(uint32_t polls)
 {

@@ -37,6 +37,8 @@ hipcstone_status enqueueFaultRead(hipStream_t s, uint32_t* host);
 hipcstone_status checkFaults(uint32_t faults);
 // the radix sort's look-back poll limit (hipcstone_set_sort_lookback_limit)
 uint32_t sortLookbackLimit();
+// waves per workgroup of the neighbor search (hipcstone_set_neighbor_waves): 2, 4 or 8
+int neighborWaves();
 
 } // namespace hipcstone
 

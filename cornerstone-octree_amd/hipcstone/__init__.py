@@ -69,6 +69,15 @@ def device_faults(stream=None):
     return int(v.value)
 
 
+def set_neighbor_waves(waves):
+    """Tuning knob: waves per workgroup of later neighbor search launches (2, 4 or 8; 0 = default 4)."""
+    _check(_lib.hipcstone_set_neighbor_waves(C.c_int32(int(waves))), "set_neighbor_waves")
+
+
+def get_neighbor_waves():
+    return int(_lib.hipcstone_get_neighbor_waves())
+
+
 def set_sort_lookback_limit(polls):
     """Test hook: polls a radix sort tile makes of an unpublished predecessor before giving up (0 = default)."""
     _check(_lib.hipcstone_set_sort_lookback_limit(C.c_uint32(int(polls))), "set_sort_lookback_limit")

@@ -1,4 +1,4 @@
-"""Autotuner of the batched clouds: the bucket size of each cloud, chosen by measurement.
+"""Autotuner: the bucket size of each batched cloud and the neighbor search's workgroup shape, chosen by measurement.
 
 The reference fixes its tree parameters per call (bucketSize) and its GPU traversal tiling at compile time
 (traversal/find_neighbors.cuh:29-58 TravConfig). Here the bucket size of every cloud of a batch is picked from
@@ -10,6 +10,10 @@ benchmark reads; `python bench.py --workload autotune` reruns the sweep and rewr
 
 A cloud's bucket size changes its tree and therefore the work of its neighbor search; the batched result of a cloud is
 bit-identical to the plain pipeline run with the same bucket size, which is what the parity tests compare.
+
+The neighbor search's waves per workgroup (2, 4 or 8; hipcstone_set_neighbor_waves) is chosen on a plain warm step
+of a Plummer sphere (the headline workload's shape at a quarter of its size), timing the neighbor stage with the
+pipeline's own HIP events. It never changes a result.
 """
 
 from __future__ import annotations
@@ -120,6 +124,52 @@ def tune(hip, clouds_per_class=8, seed=4242, target=64.0, ngmax=128, curve=None,
     return {"bucket": table, "bands": [list(b) for b in BANDS], "candidates": list(CANDIDATES),
             "sweep": sweep, "clouds_per_class": clouds_per_class, "target_neighbors": target, "ngmax": ngmax,
             "drift": drift, "measured": time.strftime("%Y-%m-%d")}
+
+
+WAVES = (2, 4, 8)
+
+
+def tune_neighbor_waves(hip, n=4_194_304, steps=4, log=print):
+    """Neighbor-stage milliseconds per warm step of a plain pipeline (fp32 Plummer a = 0.05, Hilbert-64, bucket 64,
+    ngmax 150, h for ~100 neighbors) for each workgroup shape; returns (best, {waves: ms})."""
+    from . import inputs
+
+    x, y, z, r = inputs.plummer(n, 0.05, seed=77, dtype=np.float32)
+    h = inputs.plummer_h(r, n, 0.05, 100.0, dtype=np.float32)
+    box = hip.make_box(-1.0, 1.0, np.float32)
+    pipe = hip.Pipeline(curve=hip.HILBERT, key_bits=64, real_bits=32, bucket_size=64, ngmax=150)
+    d = [hip.DeviceArray.from_numpy(a) for a in (x, y, z, h)]
+    nb = hip.DeviceArray(n * 150, np.uint32)
+    nc = hip.DeviceArray(n, np.uint32)
+    pipe.sync(*d, n, box, full_build=True, neighbors=nb, counts=nc)
+    before = hip.get_neighbor_waves()
+    times = {}
+    try:
+        for w in WAVES:
+            hip.set_neighbor_waves(w)
+            v = pipe.view()
+            pipe.sync(v.x, v.y, v.z, v.h, n, box, neighbors=nb, counts=nc)  # warm-up at this shape
+            pipe.set_timing(True)
+            for k in range(steps):
+                v = pipe.view()
+                hip.displace(v.x, v.y, v.z, v.h, n, 1e-3, 3, k, box)
+                pipe.sync(v.x, v.y, v.z, v.h, n, box, neighbors=nb, counts=nc)
+            ms, cnt = pipe.stage_times()["neighbors"]
+            pipe.set_timing(False)
+            times[w] = ms / max(cnt, 1)
+            log(f"autotune neighbor waves {w}: {times[w]:.3f} ms per search ({n} particles)")
+    finally:
+        hip.set_neighbor_waves(before)
+    return min(times, key=times.get), times
+
+
+def apply(hip, tuned=None):
+    """Applies the persisted workgroup shape of the neighbor search (if any); returns it."""
+    tuned = load() if tuned is None else tuned
+    w = int((tuned or {}).get("neighbor_waves", 0) or 0)
+    if w:
+        hip.set_neighbor_waves(w)
+    return w or hip.get_neighbor_waves()
 
 
 def save(tuned, path: str = TUNED_PATH):

@@ -91,6 +91,11 @@ hipcstone_status hipcstone_device_faults(hipcstone_stream s, uint32_t* faults);
 /* Test hook: the number of polls a radix sort tile makes of a predecessor that has not published its prefix before
  * it gives up (HIPCSTONE_FAULT_SORT_LOOKBACK); 0 restores the default (2^22). Applies to later sort launches. */
 hipcstone_status hipcstone_set_sort_lookback_limit(uint32_t polls);
+/* Tuning knob (no reference counterpart; the reference fixes its traversal tiling at compile time,
+ * traversal/find_neighbors.cuh:29-58): waves per workgroup of every later neighbor search launch, 2, 4 (default) or 8
+ * (0 restores the default). Results do not depend on it. Process-wide; the autotuner sets it. */
+hipcstone_status hipcstone_set_neighbor_waves(int32_t waves);
+int32_t hipcstone_get_neighbor_waves(void);
 /* library version and the gfx target it was built for */
 const char* hipcstone_build_info(void);
 
