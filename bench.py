@@ -505,8 +505,9 @@ def grav_cpu_stage_baseline(dom, d):
 def ijloop_bench(args):
     """ijloop over the plain pipeline's tree: the same 16,777,216 fp32 Plummer particles (h from the analytic
     density), Hilbert-64 keys, bucketSize 64, a converged tree; one step = one ijLoop of the example interaction
-    (hipcstone_ijloop.hpp: neighbor search into chunked scratch, then the per-body interaction fold over its first
-    ngmax = 150 neighbors: kernel sum, count, min / max distance, odd-symmetric x moment, postamble)."""
+    (hipcstone_ijloop.hpp: one fused kernel, each wave's neighbor walk followed by its bodies' interaction fold over
+    their first ngmax = 150 neighbors: kernel sum, count, min / max distance, odd-symmetric x moment, postamble).
+    The line also carries the neighbor search alone over the same tree, with and without its lists."""
     import hipcstone as hip
     from hipcstone import inputs
 
@@ -542,6 +543,29 @@ def ijloop_bench(args):
     per_step = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(K)])
     total_ms = float(per_step.sum())
     cnt = outs[1].download()
+
+    # the neighbor search alone over the same tree and bodies: with its lists (ngmax 150, row-major, every list
+    # written to device memory) and count-only (ngmax 0)
+    def search_ms(ng):
+        nb = hip.DeviceArray(max(n * ng, 1), np.uint32)
+        nc = hip.DeviceArray(n, np.uint32)
+
+        def run():
+            hip.find_neighbors(v.x, v.y, v.z, v.h, 0, n, box, v.child_offsets, v.parents, v.internal_to_leaf,
+                               v.layout, v.centers, v.sizes, v.num_nodes, ng, nb, nc)
+
+        run()
+        hip.device_sync()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record()
+        for _ in range(K):
+            run()
+        e1.record()
+        hip.device_sync()
+        return e0.elapsed_ms(e1) / K
+
+    search = {"with_lists_ms": round(search_ms(ngmax), 3), "count_only_ms": round(search_ms(0), 3)}
+    search["ijloop_over_count_only"] = round(total_ms / K / search["count_only_ms"], 3)
     result = {
         "metric": "Mparticles/s ijLoop (neighbor search + interaction fold, 16M fp32)",
         "value": round(n * K / (total_ms / 1e3) / 1e6, 2),
@@ -555,6 +579,7 @@ def ijloop_bench(args):
         },
         "latency_ms": {"p50": round(float(np.percentile(per_step, 50)), 3),
                        "p99": round(float(np.percentile(per_step, 99)), 3)},
+        "search_alone": search,
         "cpu_baseline": None if args.no_cpu_baseline else ijloop_cpu_baseline(v, n, m[perm], ngmax),
     }
     print(json.dumps(result), flush=True)

@@ -11,11 +11,10 @@
 #include <cstdint>
 
 #include "hipcstone.h"
+#include "hipcstone_nbwalk.hpp" // wave64 primitives, BoxOf, the neighbor walk
 
 namespace hipcstone
 {
-
-constexpr int kWave = 64;
 
 template<class K>
 struct KeyTraits;
@@ -291,19 +290,6 @@ __host__ __device__ inline void smallestCommonBox(K a, K b, K& start, K& end)
     end                  = start + nodeRange<K>(commonLevel);
 }
 
-template<class T>
-struct BoxOf;
-template<>
-struct BoxOf<float>
-{
-    using type = hipcstone_box_f32;
-};
-template<>
-struct BoxOf<double>
-{
-    using type = hipcstone_box_f64;
-};
-
 // sfc/sfc.hpp:142-179 sfc3D: ix = int(floor(x*mx) - xmin*mx) with mx = 2^L * ilx, clamped to 2^L - 1
 template<class K, class T, class Box>
 __host__ __device__ inline void sfcGridCoords(T x, T y, T z, const Box& box, int& ix, int& iy, int& iz)
@@ -330,20 +316,7 @@ __host__ __device__ inline K sfc3D(T x, T y, T z, const Box& box)
     else { return iMorton<K>(unsigned(ix), unsigned(iy), unsigned(iz)); }
 }
 
-// ---- wave64 primitives ----
-
-__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-
-// this lane's bit of a wave-uniform mask; the mask is used as a lane predicate as is (no per-lane shift and compare)
-__device__ inline bool laneIn(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
-
-// number of set bits of mask below this lane
-__device__ inline unsigned laneRank(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-}
-
-__device__ inline int laneId() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+// ---- wave64 primitives (the rest are in hipcstone_nbwalk.hpp) ----
 
 template<class T>
 __device__ inline T readLane(T v, int lane);
@@ -370,35 +343,6 @@ __device__ inline double readLane(double v, int lane)
     int lo    = __builtin_amdgcn_readlane(int(uint32_t(b)), lane);
     int hi    = __builtin_amdgcn_readlane(int(uint32_t(uint64_t(b) >> 32)), lane);
     return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
-}
-
-__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline unsigned uniform(unsigned v) { return unsigned(__builtin_amdgcn_readfirstlane(int(v))); }
-
-__device__ inline uint64_t uniform64(uint64_t v)
-{
-    uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
-    uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
-    return (uint64_t(hi) << 32) | lo;
-}
-
-// order LDS writes of some lanes before LDS reads of other lanes of the same wave
-__device__ inline void waveSync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// XCD-aware work order: workgroups are dealt round-robin to the 8 XCDs (each with its own L2); remap so that each
-// XCD walks one contiguous range of groups, whose neighbouring work items share tree nodes in its L2.
-// numBlocksPadded is a multiple of kXcds.
-constexpr uint32_t kXcds = 8;
-
-__device__ inline uint32_t xcdContiguousBlock(uint32_t b, uint32_t numBlocksPadded)
-{
-    uint32_t perXcd = numBlocksPadded / kXcds;
-    return (b % kXcds) * perXcd + b / kXcds;
 }
 
 inline unsigned ceilDiv(size_t a, size_t b) { return unsigned((a + b - 1) / b); }

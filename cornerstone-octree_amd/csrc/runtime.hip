@@ -144,6 +144,13 @@ hipcstone_status hipcstone_device_faults(hipcstone_stream s, uint32_t* faults)
     return checkFaults(v);
 }
 
+hipcstone_status hipcstone_fault_word(uint32_t** word)
+{
+    if (!word) return invalid("fault_word: null pointer");
+    *word = faultWord();
+    return *word ? HIPCSTONE_OK : HIPCSTONE_HIP_ERROR;
+}
+
 hipcstone_status hipcstone_set_neighbor_waves(int32_t waves)
 {
     if (waves == 0) waves = 4;

@@ -88,6 +88,10 @@ const char* hipcstone_last_error(void);
  * that fired since the last call, on any stream; *faults may be NULL). Returns HIPCSTONE_DEVICE_FAULT if any bit was
  * set. The pipeline and domain objects also check the word at their own synchronization points. */
 hipcstone_status hipcstone_device_faults(hipcstone_stream s, uint32_t* faults);
+/* The device address of the current device's fault word (no reference counterpart): for kernels that callers build
+ * from this library's device headers (the fused interaction loop of hipcstone_ijloop.hpp), so that their guards
+ * raise the same HIPCSTONE_FAULT_* bits that hipcstone_device_faults reports. */
+hipcstone_status hipcstone_fault_word(uint32_t** word);
 /* Test hook: the number of polls a radix sort tile makes of a predecessor that has not published its prefix before
  * it gives up (HIPCSTONE_FAULT_SORT_LOOKBACK); 0 restores the default (2^22). Applies to later sort launches. */
 hipcstone_status hipcstone_set_sort_lookback_limit(uint32_t polls);

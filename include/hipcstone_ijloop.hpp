@@ -17,12 +17,15 @@
  * folded sequentially in list order, so results equal the reference CPU ijloop's (cpu_alwaystraverse.hpp:90-113)
  * bit for bit when the interaction is evaluated with the same operations.
  *
- * The neighbor search runs in the library (libhipcstone.so); the interaction kernel is instantiated here, in the
- * caller's translation unit, for the caller's interaction and postamble (device-callable functors or lambdas).
- * Bodies are processed in chunks so that the neighbor scratch stays below 2 GiB; the scratch holds each group of 64
- * bodies' lists slot-major (hipcstone_find_neighbors_slot_major_*), so the fold's per-slot index loads coalesce, and
- * x, y, z, h of all totalBodies bodies are staged as one record per body at each ijLoop, so a neighbor's position
- * and h come in one wide load (the values are copies, the results do not change).
+ * The neighbor search and the fold run in one kernel, instantiated here, in the caller's translation unit, for the
+ * caller's interaction and postamble (device-callable functors or lambdas), as the reference's GPU loop folds each
+ * warp's lists right after that warp's traversal (gpu_alwaystraverse.cuh:32-91): a persistent grid of waves takes
+ * groups of 64 consecutive bodies; each wave walks the tree for its group with the library's own walk
+ * (hipcstone_nbwalk.hpp), writing the 64 lists into its own slice of a small scratch (64 * ngmax slots per resident
+ * wave, reused group after group: the lists are never held in memory whole), then every lane folds its own body's
+ * list from that slice, in list order. x, y, z, h of all totalBodies bodies are also staged as one
+ * record per body at each ijLoop, so a neighbor's position and h come in one wide load (the values are copies, the
+ * results do not change).
  * h must be an array (the reference also accepts a constant). Errors are thrown as std::runtime_error carrying
  * hipcstone_last_error().
  */
@@ -43,6 +46,7 @@
 #include <utility>
 
 #include "hipcstone.h"
+#include "hipcstone_nbwalk.hpp"
 
 namespace hipcstone::ijloop
 {
@@ -235,6 +239,7 @@ __device__ inline auto loadParticleData(const Record<T>* rec, std::tuple<const T
     return std::tuple<LocalIndex, Vec3<T>, T, Ts...>(i, Vec3<T>{{r.x, r.y, r.z}}, r.h, std::get<I>(input)[i]...);
 }
 
+
 template<class... Ts, class... Vs, std::size_t... I>
 __device__ inline void storeParticleData(std::tuple<Ts*...> const& output, LocalIndex i,
                                          std::tuple<Vs...> const& value, std::index_sequence<I...>)
@@ -270,51 +275,135 @@ __device__ inline void posDiffAndDistSq(bool usePbc, const Box<T>& box, const Ve
     d2 = r[0] * r[0] + (r[1] * r[1] + r[2] * r[2]);
 }
 
-template<bool AnyPbc, class T, class... In, class... Out, class Interaction, class Postamble>
-__global__ __launch_bounds__(256) void ijLoopKernel(Box<T> box, const Record<T>* __restrict__ rec,
-                                                     std::tuple<const In*...> input, std::tuple<Out*...> output,
-                                                     Interaction interaction, Postamble postamble, unsigned ngmax,
-                                                     const LocalIndex* __restrict__ neighbors,
-                                                     const LocalIndex* __restrict__ counts, LocalIndex chunkFirst,
-                                                     LocalIndex chunkLast, GroupView groups)
+// body i belongs to the subgroups: the ranges [groupStart[g], groupEnd[g]) are ascending and disjoint
+__device__ inline bool inGroups(const GroupView& groups, LocalIndex i)
 {
-    auto body = [&](LocalIndex i) {
-        const auto iData = loadParticleData(rec, input, i, std::index_sequence_for<In...>{});
-        const Vec3<T> pi = std::get<1>(iData);
-        bool usePbc      = false;
-        if constexpr (AnyPbc) usePbc = requiresPbc(box, pi, std::get<2>(iData));
-        const LocalIndex row = i - chunkFirst;
-        const unsigned nbs   = counts[row] < ngmax ? counts[row] : ngmax;
-        // slot-major scratch: neighbor nb of row r at ((r / 64) * ngmax + nb) * 64 + r % 64
-        const LocalIndex* slots = neighbors + size_t(row >> 6) * ngmax * 64 + (row & 63u);
-        auto result          = interaction(iData, iData, Vec3<T>{{T(0), T(0), T(0)}}, T(0));
-        constexpr auto idx   = std::make_index_sequence<std::tuple_size_v<decltype(result)>>{};
-        for (unsigned nb = 0; nb < nbs; ++nb)
-        {
-            const LocalIndex j = slots[size_t(nb) * 64];
-            const auto jData   = loadParticleData(rec, input, j, std::index_sequence_for<In...>{});
-            Vec3<T> r;
-            T d2;
-            posDiffAndDistSq(usePbc, box, pi, std::get<1>(jData), r, d2);
-            updateResult(result, interaction(iData, jData, r, d2), idx);
-        }
-        const auto out = postamble(iData, unwrapModifiers(result, idx));
-        storeParticleData(output, i, out, std::index_sequence_for<Out...>{});
-    };
-
-    if (groups.numGroups == 0)
+    if (groups.numGroups == 0) return true;
+    LocalIndex lo = 0, hi = groups.numGroups; // the first range that starts after i
+    while (lo < hi)
     {
-        LocalIndex i = chunkFirst + LocalIndex(blockIdx.x) * blockDim.x + threadIdx.x;
-        if (i < chunkLast) body(i);
-        return;
+        LocalIndex mid = lo + (hi - lo) / 2;
+        if (groups.groupStart[mid] <= i) lo = mid + 1;
+        else hi = mid;
     }
-    // subgroups: block g walks group g's bodies inside this chunk
-    for (LocalIndex g = blockIdx.x; g < groups.numGroups; g += gridDim.x)
+    return lo > 0 && i < groups.groupEnd[lo - 1];
+}
+
+constexpr int kFusedWaves     = 4; // waves per workgroup of the fused kernel
+constexpr unsigned kFoldBatch = 8; // neighbors whose data a fold step gathers at once
+
+__global__ void zeroCountersKernel(uint32_t* counters)
+{
+    if (threadIdx.x < hipcstone::kXcds) counters[threadIdx.x] = 0;
+}
+
+/*
+ * The fused always-traverse loop. Bodies [first, last) form groups of 64; the groups are split into 8 contiguous
+ * ranges, one per XCD (workgroups are dealt to the XCDs round-robin, so workgroup b runs on XCD b % 8 and neighbouring
+ * groups share tree nodes and particles in that XCD's L2). Each wave takes the next group of its XCD's range from
+ * counters[xcd] and, when that range is exhausted, helps with the next XCD's. Per group: the walk stores the 64 lists
+ * (first ngmax neighbors, ascending j) row-major into the wave's scratch slice (64 * ngmax slots, reused group after
+ * group), the wave's stores are made visible to its own lanes, each lane folds its body's list in order and the
+ * postamble result is stored. Bodies outside the subgroups take no part.
+ */
+template<bool AnyPbc, class T, class InTuple, class OutTuple, class Interaction, class Postamble>
+__global__ __launch_bounds__(kFusedWaves * 64) void fusedIjLoopKernel(
+    Box<T> box, const int32_t* __restrict__ childOffsets, const int32_t* __restrict__ internalToLeaf,
+    const LocalIndex* __restrict__ layout, const T* __restrict__ centers, const T* __restrict__ sizes,
+    int32_t numNodes, float ext, const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+    const T* __restrict__ h, const Record<T>* __restrict__ rec, InTuple input, OutTuple output,
+    Interaction interaction, Postamble postamble, unsigned ngmax, LocalIndex first, LocalIndex last,
+    GroupView groups, LocalIndex* __restrict__ scratch, uint32_t* __restrict__ counters,
+    uint32_t* __restrict__ faults)
+{
+    using namespace hipcstone;
+    // the tree's arrays come in as separate restrict-qualified arguments: read-only memory that nothing aliases,
+    // so the walk's wave-uniform node reads become scalar loads
+    const nbwalk::Tree<T> tree{childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext};
+    constexpr auto inIdx = std::make_index_sequence<std::tuple_size_v<InTuple>>{};
+    __shared__ nbwalk::Shared<T, kFusedWaves> sh;
+    const int w             = uniform(int(threadIdx.x) / kWave);
+    const int lane          = laneId();
+    LocalIndex* rows        = scratch + (size_t(blockIdx.x) * kFusedWaves + size_t(w)) * kWave * ngmax;
+    const uint32_t rowBytes = uint32_t(kWave) * ngmax * uint32_t(sizeof(LocalIndex));
+    const uint32_t numGrp   = uint32_t((uint64_t(last) - first + kWave - 1) / kWave);
+    const uint32_t xcd      = blockIdx.x % kXcds;
+
+    for (uint32_t r = 0; r < kXcds; ++r)
     {
-        LocalIndex b = groups.groupStart[g] > chunkFirst ? groups.groupStart[g] : chunkFirst;
-        LocalIndex e = groups.groupEnd[g] < chunkLast ? groups.groupEnd[g] : chunkLast;
-        for (LocalIndex i = b + threadIdx.x; i < e; i += blockDim.x)
-            body(i);
+        const uint32_t part = (xcd + r) % kXcds;
+        const uint32_t gb   = uint32_t(uint64_t(numGrp) * part / kXcds);
+        const uint32_t ge   = uint32_t(uint64_t(numGrp) * (part + 1) / kXcds);
+        while (true)
+        {
+            uint32_t g = 0;
+            if (lane == 0) g = atomicAdd(&counters[part], 1u);
+            g = uniform(g);
+            if (g >= ge - gb) break;
+            const uint64_t gFirst = uint64_t(first) + uint64_t(gb + g) * kWave;
+            const LocalIndex i    = LocalIndex(gFirst) + LocalIndex(lane);
+            const bool valid      = i < last && inGroups(groups, i);
+            if (ballot(valid) == 0) continue;
+
+            // the walk writes the 64 lists (first ngmax neighbors, ascending j) row-major into the wave's slice
+            const uint32_t count = nbwalk::walk<T, AnyPbc, false, false, kFusedWaves>(
+                x, y, z, h, box, tree, 0, gFirst, valid, ngmax, ngmax ? rows : nullptr, rowBytes, sh, w, faults);
+            // the lists were stored by the lanes that tested each pair: make them visible to the lanes that fold them
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+            // each lane folds its body's list: the self term first, then the neighbors in list order. Lanes without
+            // a body fold an empty list of body `first` and store nothing.
+            const LocalIndex iOwn = valid ? i : first;
+            const auto iData      = loadParticleData(rec, input, iOwn, inIdx);
+            const Vec3<T> pi      = std::get<1>(iData);
+            bool usePbc           = false;
+            if constexpr (AnyPbc) usePbc = requiresPbc(box, pi, std::get<2>(iData));
+            auto result           = interaction(iData, iData, Vec3<T>{{T(0), T(0), T(0)}}, T(0));
+            constexpr auto idx    = std::make_index_sequence<std::tuple_size_v<decltype(result)>>{};
+            const unsigned stored = valid ? (count < ngmax ? count : ngmax) : 0u;
+            // the list is read kFoldBatch neighbors at a time, the next batch's indices loaded while the current
+            // batch's data is gathered, so a batch waits for one gather instead of two dependent loads per neighbor;
+            // slots past the list load the body itself (a valid address) and are not folded
+            const LocalIndex* myRow = rows + size_t(lane) * ngmax;
+            auto slotIndex          = [&](unsigned k) { return k < stored ? myRow[k] : iOwn; };
+            LocalIndex nextJ[kFoldBatch];
+#pragma unroll
+            for (unsigned u = 0; u < kFoldBatch; ++u)
+                nextJ[u] = slotIndex(u);
+            for (unsigned nb = 0; nb < stored; nb += kFoldBatch)
+            {
+                LocalIndex curJ[kFoldBatch];
+#pragma unroll
+                for (unsigned u = 0; u < kFoldBatch; ++u)
+                {
+                    curJ[u]  = nextJ[u];
+                    nextJ[u] = slotIndex(nb + kFoldBatch + u);
+                }
+                decltype(loadParticleData(rec, input, iOwn, inIdx)) jData[kFoldBatch];
+#pragma unroll
+                for (unsigned u = 0; u < kFoldBatch; ++u)
+                    jData[u] = loadParticleData(rec, input, curJ[u], inIdx);
+#pragma unroll
+                for (unsigned u = 0; u < kFoldBatch; ++u)
+                {
+                    if (nb + u < stored)
+                    {
+                        Vec3<T> rij;
+                        T d2;
+                        posDiffAndDistSq(usePbc, box, pi, std::get<1>(jData[u]), rij, d2);
+                        updateResult(result, interaction(iData, jData[u], rij, d2), idx);
+                    }
+                }
+            }
+            // (every load of the slice above has completed, its value used, before the next walk writes the slice)
+            if (valid)
+            {
+                const auto out = postamble(iData, unwrapModifiers(result, idx));
+                storeParticleData(output, i, out, std::make_index_sequence<std::tuple_size_v<OutTuple>>{});
+            }
+        }
     }
 }
 
@@ -333,34 +422,6 @@ struct DeviceFree
     void operator()(void* p) const { (void)hipFree(p); }
 };
 
-inline hipcstone_status findNeighbors(hipStream_t s, const float* x, const float* y, const float* z, const float* h,
-                                      LocalIndex first, LocalIndex last, const Box<float>& box,
-                                      const int32_t* childOffsets, const int32_t* parents,
-                                      const int32_t* internalToLeaf, const LocalIndex* layout, const void* centers,
-                                      const void* sizes, int32_t numNodes, float ext, unsigned ngmax,
-                                      LocalIndex* nb, LocalIndex* nc)
-{
-    return hipcstone_find_neighbors_slot_major_f32(reinterpret_cast<hipcstone_stream>(s), x, y, z, h, first, last,
-                                                   &box,
-                                        childOffsets, parents, internalToLeaf, layout,
-                                        static_cast<const float*>(centers), static_cast<const float*>(sizes),
-                                        numNodes, ext, ngmax, nb, nc);
-}
-
-inline hipcstone_status findNeighbors(hipStream_t s, const double* x, const double* y, const double* z,
-                                      const double* h, LocalIndex first, LocalIndex last, const Box<double>& box,
-                                      const int32_t* childOffsets, const int32_t* parents,
-                                      const int32_t* internalToLeaf, const LocalIndex* layout, const void* centers,
-                                      const void* sizes, int32_t numNodes, float ext, unsigned ngmax,
-                                      LocalIndex* nb, LocalIndex* nc)
-{
-    return hipcstone_find_neighbors_slot_major_f64(reinterpret_cast<hipcstone_stream>(s), x, y, z, h, first, last,
-                                                   &box,
-                                        childOffsets, parents, internalToLeaf, layout,
-                                        static_cast<const double*>(centers), static_cast<const double*>(sizes),
-                                        numNodes, ext, ngmax, nb, nc);
-}
-
 } // namespace detail
 
 // gpu_alwaystraverse.cuh:99-178
@@ -368,7 +429,7 @@ template<class T, class KeyType>
 class GpuAlwaysTraverseNeighborhood
 {
 public:
-    static constexpr std::size_t kScratchElements = std::size_t(1) << 29; // neighbor slots per chunk (2 GiB)
+    static constexpr std::size_t kScratchBytes = std::size_t(1) << 31; // the neighbor scratch's cap
 
     // totalBodies: the length of x, y, z, h (neighbors lie anywhere in it)
     GpuAlwaysTraverseNeighborhood(hipStream_t stream, const OctreeNsView<T, KeyType>& tree, const Box<T>& box,
@@ -377,19 +438,11 @@ public:
         : stream_(stream), tree_(tree), box_(box), groups_(groups), x_(x), y_(y), z_(z), h_(h), ngmax_(ngmax),
           total_(std::max(totalBodies, groups.lastBody))
     {
-        LocalIndex bodies = groups.lastBody > groups.firstBody ? groups.lastBody - groups.firstBody : 0;
-        std::size_t fit   = std::max<std::size_t>(kScratchElements / std::max(ngmax, 1u) / 64 * 64, 64);
-        chunk_            = LocalIndex(std::min<std::size_t>((std::size_t(std::max<LocalIndex>(bodies, 1)) + 63) / 64 * 64,
-                                                             fit));
-        void* p           = nullptr;
-        detail::checkHip(hipMalloc(&p, std::max<std::size_t>(std::size_t(chunk_) * ngmax, 1) * sizeof(LocalIndex)),
-                         "ijloop: neighbor scratch");
-        neighbors_.reset(static_cast<LocalIndex*>(p));
-        detail::checkHip(hipMalloc(&p, std::size_t(chunk_) * sizeof(LocalIndex)), "ijloop: count scratch");
-        counts_.reset(static_cast<LocalIndex*>(p));
-        detail::checkHip(hipMalloc(&p, std::max<std::size_t>(total_, 1) * sizeof(detail::Record<T>)),
-                         "ijloop: record scratch");
-        records_.reset(static_cast<detail::Record<T>*>(p));
+        if (ngmax > (1u << 20)) throw std::runtime_error("ijloop: ngmax above 2^20");
+        void* p = nullptr;
+        detail::checkHip(hipMalloc(&p, hipcstone::kXcds * sizeof(uint32_t)), "ijloop: group counters");
+        counters_.reset(static_cast<uint32_t*>(p));
+        detail::check(hipcstone_fault_word(&faults_), "ijloop: fault word");
     }
 
     template<class... In, class... Out, class Interaction, class Postamble>
@@ -403,8 +456,7 @@ public:
     Statistics stats() const
     {
         return {std::size_t(groups_.lastBody - groups_.firstBody),
-                std::size_t(chunk_) * (std::size_t(ngmax_) + 1) * sizeof(LocalIndex) +
-                    std::size_t(total_) * sizeof(detail::Record<T>)};
+                scratchBytes_ + recordBytes_ + hipcstone::kXcds * sizeof(uint32_t)};
     }
 
     struct Subgroup
@@ -435,29 +487,62 @@ private:
         std::tuple<const std::remove_const_t<In>*...> cin = input;
         const bool anyPbc = box_.bnd[0] == HIPCSTONE_PERIODIC || box_.bnd[1] == HIPCSTONE_PERIODIC ||
                             box_.bnd[2] == HIPCSTONE_PERIODIC;
-        detail::stageRecordsKernel<T><<<unsigned((std::size_t(total_) + 255) / 256), 256, 0, stream_>>>(
-            x_, y_, z_, h_, total_, records_.get());
-        detail::checkHip(hipGetLastError(), "ijloop: record staging");
-        for (LocalIndex c0 = first; c0 < last; c0 += std::min(chunk_, last - c0))
+        if (anyPbc) launch<true>(cin, output, interaction, postamble, groups, first, last);
+        else launch<false>(cin, output, interaction, postamble, groups, first, last);
+    }
+
+    template<bool AnyPbc, class InTuple, class OutTuple, class Interaction, class Postamble>
+    void launch(InTuple const& input, OutTuple const& output, Interaction const& interaction,
+                Postamble const& postamble, GroupView groups, LocalIndex first, LocalIndex last) const
+    {
+        constexpr int kThreads = detail::kFusedWaves * 64;
+        auto kernel = &detail::fusedIjLoopKernel<AnyPbc, T, InTuple, OutTuple, Interaction, Postamble>;
+        // a persistent grid: as many workgroups as fit on the device at once (whole rounds over the 8 XCDs), no more
+        // than the groups of 64 bodies need, and no more than a 2 GiB scratch holds (each wave owns a slice of
+        // 64 * ngmax neighbor slots)
+        int device = 0, cus = 0, perCu = 0;
+        detail::checkHip(hipGetDevice(&device), "ijloop: device");
+        detail::checkHip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "ijloop: CUs");
+        detail::checkHip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, kThreads, 0),
+                         "ijloop: occupancy");
+        const std::size_t groups64  = (std::size_t(last - first) + 63) / 64;
+        const std::size_t waveBytes = std::size_t(64) * ngmax_ * sizeof(LocalIndex);
+        std::size_t blocks          = std::size_t(std::max(perCu, 1)) * std::size_t(std::max(cus, 1));
+        blocks = std::min(blocks, (groups64 + detail::kFusedWaves - 1) / detail::kFusedWaves);
+        if (waveBytes) blocks = std::min(blocks, std::max<std::size_t>(kScratchBytes / waveBytes / detail::kFusedWaves, 1));
+        if (blocks > hipcstone::kXcds) blocks = blocks / hipcstone::kXcds * hipcstone::kXcds;
+        const std::size_t need = blocks * detail::kFusedWaves * waveBytes;
+        if (need > scratchBytes_)
         {
-            const LocalIndex c1 = c0 + std::min(chunk_, last - c0);
-            detail::check(detail::findNeighbors(stream_, x_, y_, z_, h_, c0, c1, box_, tree_.childOffsets,
-                                                tree_.parents, tree_.internalToLeaf, tree_.layout, tree_.centers,
-                                                tree_.sizes, tree_.numNodes, tree_.searchExtFactor, ngmax_,
-                                                neighbors_.get(), counts_.get()),
-                          "ijloop: findNeighbors");
-            unsigned blocks = groups.numGroups ? std::min<unsigned>(groups.numGroups, 65535u)
-                                               : unsigned((std::size_t(c1 - c0) + 255) / 256);
-            if (anyPbc)
-                detail::ijLoopKernel<true, T><<<blocks, 256, 0, stream_>>>(
-                    box_, records_.get(), cin, output, interaction, postamble, ngmax_, neighbors_.get(),
-                    counts_.get(), c0, c1, groups);
-            else
-                detail::ijLoopKernel<false, T><<<blocks, 256, 0, stream_>>>(
-                    box_, records_.get(), cin, output, interaction, postamble, ngmax_, neighbors_.get(),
-                    counts_.get(), c0, c1, groups);
-            detail::checkHip(hipGetLastError(), "ijloop: interaction kernel");
+            scratch_.reset();
+            scratchBytes_ = 0;
+            void* p       = nullptr;
+            detail::checkHip(hipMalloc(&p, need), "ijloop: neighbor scratch");
+            scratch_.reset(static_cast<LocalIndex*>(p));
+            scratchBytes_ = need;
         }
+        const std::size_t recBytes = std::max<std::size_t>(total_, 1) * sizeof(detail::Record<T>);
+        if (recBytes > recordBytes_)
+        {
+            records_.reset();
+            recordBytes_ = 0;
+            void* p      = nullptr;
+            detail::checkHip(hipMalloc(&p, recBytes), "ijloop: record scratch");
+            records_.reset(p);
+            recordBytes_ = recBytes;
+        }
+        auto* rec = static_cast<detail::Record<T>*>(records_.get());
+        detail::stageRecordsKernel<T><<<unsigned((std::size_t(total_) + 255) / 256), 256, 0, stream_>>>(
+            x_, y_, z_, h_, total_, rec);
+        detail::checkHip(hipGetLastError(), "ijloop: record staging");
+        detail::zeroCountersKernel<<<1, 64, 0, stream_>>>(counters_.get());
+        detail::checkHip(hipGetLastError(), "ijloop: group counters");
+        kernel<<<unsigned(blocks), kThreads, 0, stream_>>>(
+            box_, tree_.childOffsets, tree_.internalToLeaf, tree_.layout, reinterpret_cast<const T*>(tree_.centers),
+            reinterpret_cast<const T*>(tree_.sizes), tree_.numNodes, tree_.searchExtFactor, x_, y_, z_, h_,
+            rec, input, output, interaction, postamble, ngmax_, first, last, groups, scratch_.get(),
+            counters_.get(), faults_);
+        detail::checkHip(hipGetLastError(), "ijloop: fused search and interaction kernel");
     }
 
     hipStream_t stream_;
@@ -467,9 +552,12 @@ private:
     const T *x_, *y_, *z_, *h_;
     unsigned ngmax_;
     LocalIndex total_;
-    LocalIndex chunk_ = 1;
-    std::unique_ptr<LocalIndex, detail::DeviceFree> neighbors_, counts_;
-    std::unique_ptr<detail::Record<T>, detail::DeviceFree> records_;
+    uint32_t* faults_ = nullptr;
+    mutable std::size_t scratchBytes_ = 0;
+    mutable std::unique_ptr<LocalIndex, detail::DeviceFree> scratch_;
+    std::unique_ptr<uint32_t, detail::DeviceFree> counters_;
+    mutable std::size_t recordBytes_ = 0;
+    mutable std::unique_ptr<void, detail::DeviceFree> records_;
 };
 
 // gpu_alwaystraverse.cuh:180-206

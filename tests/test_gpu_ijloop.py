@@ -57,7 +57,8 @@ def test_ijloop_example_gpu(hip, oracle, T, bnd, ngmax, first, last, ext):
 
 
 def test_ijloop_example_gpu_chunked(hip, oracle):
-    """ngmax = 65536 caps the neighbor scratch at 8192 bodies per chunk, so 30000 bodies take four chunks."""
+    """ngmax = 65536: a wave's scratch slice is 16 MiB, so the 2 GiB scratch cap leaves 128 waves, and each wave
+    walks and folds several of the 469 groups of 64 bodies."""
     T = np.float32
     print("ijloop chunked", flush=True)
     p, obox, m = _case(oracle, T, 30000, (0, 0, 0), seed=13)
