@@ -21,8 +21,16 @@ namespace hipcstone
 // Waves per workgroup: 4 by default; 2 and 8 are the other shapes the autotuner measures
 // (hipcstone_set_neighbor_waves). Every wave owns its own targets, stack and leaf list, so the shape changes only
 // how waves are grouped onto CUs (LDS per workgroup, scheduling granularity), never a result.
+// The fp32 open-box walk fits 8 waves per SIMD when its scalar registers are capped (the compiler then keeps some
+// wave-uniform values of the walk in vector-register lanes): more waves to cover the latency of its LDS, scalar-load
+// and compare-to-ballot dependencies, measured 6 % faster than the 7 waves its uncapped allocation allows. The other
+// variants keep the compiler's own allocation.
+template<class T, bool AnyPbc>
+constexpr int kNbMinWavesPerEu = std::is_same_v<T, float> && !AnyPbc ? 8 : 1;
+
 template<class T, bool AnyPbc, bool SlotMajor, bool Segmented, int kNbWaves>
-__global__ __launch_bounds__(kNbWaves * kWave) void findNeighborsKernel(
+__global__ __launch_bounds__(kNbWaves * kWave) __attribute__((amdgpu_waves_per_eu(kNbMinWavesPerEu<T, AnyPbc>, 8)))
+void findNeighborsKernel(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, const T* __restrict__ h,
     uint32_t firstId, uint32_t lastId, typename BoxOf<T>::type box, const int32_t* __restrict__ childOffsets,
     const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
