@@ -141,9 +141,14 @@ __device__ inline uint32_t addLaneBit(uint32_t v, uint64_t mask)
     return r;
 }
 
-// dword 3 of a raw (untyped, unswizzled) buffer resource on gfx9-family targets: 32-bit data format, range checked
-// against the byte count in dword 2
-constexpr int kRawBufferFlags = 0x00020000;
+// the structured buffer store of the LLVM AMDGPU back end (address = base + vindex * stride; with stride != 0 a
+// vindex at or past the record count is out of range and the store is dropped), which has no clang builtin
+__device__ void structBufferStoreB32(uint32_t data, __amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                     int aux) __asm("llvm.amdgcn.struct.ptr.buffer.store.i32");
+
+// dword 3 of an untyped, unswizzled buffer resource on gfx9-family targets (32-bit data format); with a stride in
+// dword 1, accesses are range checked against the record count in dword 2
+constexpr int kBufferFlags = 0x00020000;
 
 template<class T>
 struct alignas(4 * sizeof(T)) Particle4
@@ -266,10 +271,11 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                              uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)) | (uint32_t(lane) << 8), 0u};
     uint32_t myWpos = rowOff;
 
-    // the wave's output rows as a raw buffer: stores at an offset past its end are dropped by the hardware's range
-    // check, so a lane without a hit stores nowhere instead of branching around the store
+    // the wave's output rows as a structured buffer of 4-byte records: a store takes the row slot itself as its
+    // index, and a lane without a hit stores at the record count, past the end, where the hardware's range check
+    // drops it (no branch around the store, no byte-offset arithmetic)
     const __amdgpu_buffer_rsrc_t rows = __builtin_amdgcn_make_buffer_rsrc(
-        rowsBytes ? static_cast<void*>(rowsPtr) : nullptr, 0, int(rowsBytes), kRawBufferFlags);
+        rowsBytes ? static_cast<void*>(rowsPtr) : nullptr, 4, int(rowsBytes / 4u), kBufferFlags);
 
     // ---- leaf phase. A leaf chunk of nj <= 32 particles is tested against its active targets in pair slots:
     //      lane = (slot, jj) with slot = lane / nj, jj = lane % nj, S = 64 / nj slots; slot s of an iteration tests
@@ -379,14 +385,14 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         if constexpr (decltype(checkSelf)::value) { H &= ballot(jRel != (rec.meta >> 8)); }
                         return H;
                     };
-                    // store byte offset of a lane's pair: its row slot for a hit (inside the row), else past the
-                    // end of the buffer, where the store is dropped
+                    // store index of a lane's pair: its row slot for a hit (inside the row), else the record count,
+                    // past the end of the buffer, where the store is dropped
                     auto storeOffset = [&](const TargetRec<T>& rec, uint64_t H, uint32_t pos) -> uint32_t {
                         if constexpr (decltype(checkLimit)::value)
                         {
-                            return (laneIn(H) & (pos < rec.limit)) ? pos * 4u : rowsBytes;
+                            return (laneIn(H) & (pos < rec.limit)) ? pos : rowsBytes / 4u;
                         }
-                        else { return laneIn(H) ? pos * 4u : rowsBytes; }
+                        else { return laneIn(H) ? pos : rowsBytes / 4u; }
                     };
                     // slot s of an iteration holds targets k0 + s (A) and k0 + S + s (B). In the iterations with
                     // k0 + 2S <= pc every slot holds a target (lanes past the last slot test a NaN particle, which
@@ -402,8 +408,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         const uint64_t HB       = hits(recB, inB);
                         const uint32_t posA     = position(HA, recA.wpos);
                         const uint32_t posB     = position(HB, recB.wpos);
-                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0);
-                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(storeOffset(recB, HB, posB)), 0, 0);
+                        structBufferStoreB32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0, 0);
+                        structBufferStoreB32(j, rows, int(storeOffset(recB, HB, posB)), 0, 0, 0);
                         if (laneIn(wA)) rA->wpos = advance(posA, HA);
                         if (laneIn(wB)) rB->wpos = advance(posB, HB);
                     };
@@ -411,7 +417,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         const TargetRec<T> recA = *rA;
                         const uint64_t HA       = hits(recA, inA);
                         const uint32_t posA     = position(HA, recA.wpos);
-                        __builtin_amdgcn_raw_buffer_store_b32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0);
+                        structBufferStoreB32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0, 0);
                         if (laneIn(wA)) rA->wpos = advance(posA, HA);
                     };
                     TargetRec<T>* rA = recs + slot; // index < 128: pc <= 64 and slot <= S <= 64
