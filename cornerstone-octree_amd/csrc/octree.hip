@@ -9,39 +9,56 @@
 #include "scan.hpp"
 
 extern "C" size_t hipcstone_sort_by_key_workspace_size_u32(size_t n);
-extern "C" size_t hipcstone_sort_by_key_workspace_size_u64(size_t n);
 
 namespace hipcstone
 {
 
+// createUnsortedLayout (octree.hpp:77-100): leaf tid is node tid + numInternal of the unsorted layout and, when it
+// and the next leaf share a prefix of whole octal digits, creates internal node (tid + binaryKeyWeight) / 7. Besides
+// the unsorted prefixes, every leaf emits its nodes' levels in creation order: entry 2 tid the internal node's (255
+// when tid creates none), entry 2 tid + 1 its own. Within one level, creation order is key order (keys ascend with
+// tid, and a level holds no node twice), so a stable sort of these entries by level alone gives the level-major,
+// key-ordered node order that sorting the prefixes gives (a level's prefixes are exactly the ones with their
+// placeholder bit at 3 * level).
 template<class K>
-__global__ __launch_bounds__(256) void unsortedLayoutKernel(const K* __restrict__ leaves, int32_t numInternal,
-                                                            int32_t numLeaves, K* __restrict__ prefixes,
-                                                            uint32_t* __restrict__ order)
+__global__ __launch_bounds__(256) void nodeLevelsKernel(const K* __restrict__ leaves, int32_t numInternal,
+                                                        int32_t numLeaves, K* __restrict__ unsortedPrefixes,
+                                                        uint32_t* __restrict__ level, uint32_t* __restrict__ node)
 {
     int32_t tid = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (tid >= numLeaves) return;
-    K key                        = leaves[tid];
-    K next                       = leaves[tid + 1];
-    unsigned level               = treeLevel(K(next - key));
-    prefixes[tid + numInternal]  = encodePlaceholderBit(key, int(3 * level));
-    order[tid + numInternal]     = uint32_t(tid + numInternal);
-    unsigned prefixLength        = unsigned(commonPrefix(key, next));
+    K key                               = leaves[tid];
+    K next                              = leaves[tid + 1];
+    unsigned leafLevel                  = treeLevel(K(next - key));
+    unsortedPrefixes[tid + numInternal] = encodePlaceholderBit(key, int(3 * leafLevel));
+    level[2 * tid + 1]                  = leafLevel;
+    node[2 * tid + 1]                   = uint32_t(tid + numInternal);
+    unsigned prefixLength               = unsigned(commonPrefix(key, next));
     if (prefixLength % 3 == 0 && tid < numLeaves - 1)
     {
-        int32_t octIndex   = (tid + binaryKeyWeight(key, prefixLength / 3)) / 7;
-        prefixes[octIndex] = encodePlaceholderBit(key, int(prefixLength));
-        order[octIndex]    = uint32_t(octIndex);
+        int32_t octIndex           = (tid + binaryKeyWeight(key, prefixLength / 3)) / 7;
+        unsortedPrefixes[octIndex] = encodePlaceholderBit(key, int(prefixLength));
+        level[2 * tid]             = prefixLength / 3;
+        node[2 * tid]              = uint32_t(octIndex);
+    }
+    else
+    {
+        level[2 * tid] = 255u;
+        node[2 * tid]  = ~0u;
     }
 }
 
-// leafToInternal[internalToLeaf[i]] = i; internalToLeaf[i] -= numInternal (octree.hpp:176-185)
+// order = the first numNodes sorted entries: prefixes[i] = unsorted prefix of node order[i];
+// leafToInternal[order[i]] = i; internalToLeaf[i] = order[i] - numInternal (octree.hpp:176-185)
+template<class K>
 __global__ void invertOrderKernel(const uint32_t* __restrict__ order, int32_t numNodes, int32_t numInternal,
+                                  const K* __restrict__ unsortedPrefixes, K* __restrict__ prefixes,
                                   int32_t* __restrict__ internalToLeaf, int32_t* __restrict__ leafToInternal)
 {
     int32_t i = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= numNodes) return;
     int32_t o          = int32_t(order[i]);
+    prefixes[i]        = unsortedPrefixes[o];
     leafToInternal[o]  = i;
     internalToLeaf[i]  = o - numInternal;
 }
@@ -134,24 +151,17 @@ __global__ __launch_bounds__(256) void geoCentersKernel(const K* __restrict__ pr
 
 extern "C" hipcstone_status hipcstone_sort_by_key_u32(hipcstone_stream, uint32_t*, uint32_t*, size_t, uint32_t*,
                                                       uint32_t*, void*, size_t, int);
-extern "C" hipcstone_status hipcstone_sort_by_key_u64(hipcstone_stream, uint64_t*, uint32_t*, size_t, uint64_t*,
-                                                      uint32_t*, void*, size_t, int);
 
 static inline size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
 
-template<class K>
-static size_t sortWs(size_t n)
-{
-    if constexpr (sizeof(K) == 4) return hipcstone_sort_by_key_workspace_size_u32(n);
-    else return hipcstone_sort_by_key_workspace_size_u64(n);
-}
-
-// workspace: [order u32 (N)][keyBuf K (N)][valueBuf u32 (N)][sort workspace]
+// workspace: [unsorted prefixes K (N)][levels u32 (2L)][nodes u32 (2L)][key buffer u32 (2L)][value buffer u32 (2L)]
+// [sort workspace of 2L u32 keys]
 template<class K>
 static size_t buildWorkspace(int32_t numLeaves)
 {
     size_t N = size_t(numLeaves) + size_t(numLeaves - 1) / 7;
-    return a256(N * 4) + a256(N * sizeof(K)) + a256(N * 4) + sortWs<K>(N);
+    size_t E = 2 * size_t(numLeaves);
+    return a256(N * sizeof(K)) + 4 * a256(E * 4) + hipcstone_sort_by_key_workspace_size_u32(E);
 }
 
 template<class K>
@@ -168,23 +178,27 @@ static hipcstone_status buildOctree(hipcstone_stream s, const K* leaves, int32_t
     int32_t numInternal = (numLeaves - 1) / 7;
     int32_t numNodes    = numInternal + numLeaves;
     size_t N            = size_t(numNodes);
+    size_t E            = 2 * size_t(numLeaves);
     char* w             = static_cast<char*>(ws);
-    uint32_t* order     = reinterpret_cast<uint32_t*>(w);
-    K* keyBuf           = reinterpret_cast<K*>(w + a256(N * 4));
-    uint32_t* valueBuf  = reinterpret_cast<uint32_t*>(w + a256(N * 4) + a256(N * sizeof(K)));
-    void* sortWsPtr     = w + a256(N * 4) + a256(N * sizeof(K)) + a256(N * 4);
-    size_t sortWsBytes  = wsBytes - (a256(N * 4) + a256(N * sizeof(K)) + a256(N * 4));
+    K* unsortedPrefixes = reinterpret_cast<K*>(w);
+    w += a256(N * sizeof(K));
+    uint32_t* level = reinterpret_cast<uint32_t*>(w);
+    w += a256(E * 4);
+    uint32_t* order = reinterpret_cast<uint32_t*>(w); // the node of each entry, then (sorted) the node order
+    w += a256(E * 4);
+    uint32_t* keyBuf = reinterpret_cast<uint32_t*>(w);
+    w += a256(E * 4);
+    uint32_t* valueBuf = reinterpret_cast<uint32_t*>(w);
+    w += a256(E * 4);
+    const size_t sortWsBytes = wsBytes - size_t(w - static_cast<char*>(ws));
 
-    unsortedLayoutKernel<K><<<ceilDiv(size_t(numLeaves), 256), 256, 0, st>>>(leaves, numInternal, numLeaves,
-                                                                              prefixes, order);
-    HC_CHECK_LAUNCH("unsortedLayoutKernel");
-    hipcstone_status e;
-    if constexpr (sizeof(K) == 4)
-        e = hipcstone_sort_by_key_u32(st, prefixes, order, N, keyBuf, valueBuf, sortWsPtr, sortWsBytes, 0);
-    else
-        e = hipcstone_sort_by_key_u64(st, prefixes, order, N, keyBuf, valueBuf, sortWsPtr, sortWsBytes, 0);
-    if (e) return e;
-    invertOrderKernel<<<ceilDiv(N, 256), 256, 0, st>>>(order, numNodes, numInternal, internalToLeaf, leafToInternal);
+    nodeLevelsKernel<K><<<ceilDiv(size_t(numLeaves), 256), 256, 0, st>>>(leaves, numInternal, numLeaves,
+                                                                          unsortedPrefixes, level, order);
+    HC_CHECK_LAUNCH("nodeLevelsKernel");
+    // one 8-bit pass: levels are at most 21, 255 marks the entries without a node (they sort last)
+    if (auto e = hipcstone_sort_by_key_u32(st, level, order, E, keyBuf, valueBuf, w, sortWsBytes, 8)) return e;
+    invertOrderKernel<K><<<ceilDiv(N, 256), 256, 0, st>>>(order, numNodes, numInternal, unsortedPrefixes, prefixes,
+                                                          internalToLeaf, leafToInternal);
     HC_CHECK_LAUNCH("invertOrderKernel");
     levelRangeKernel<K><<<1, 64, 0, st>>>(prefixes, numNodes, levelRange);
     HC_CHECK_LAUNCH("levelRangeKernel");
