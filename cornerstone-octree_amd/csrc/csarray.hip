@@ -125,8 +125,9 @@ __global__ __launch_bounds__(256) void rebalanceKernel(const K* __restrict__ tre
 }
 
 __global__ void setIntKernel(int* p, int v) { *p = v; }
+__global__ void copyI32Kernel(const int32_t* src, int32_t* dst) { *dst = *src; }
 
-// workspace: [flag int (256 B)][scan partials]
+// workspace: [flag int, new node count int (256 B)][scan partials]
 static size_t nodeOpsWorkspace(int32_t numNodes) { return 256 + scan::workspaceBytes(size_t(numNodes) + 1); }
 
 template<class K>
@@ -161,12 +162,15 @@ static hipcstone_status computeNodeOps(hipcstone_stream s, const K* tree, int32_
     if (auto e = scan::exclusiveScan<int32_t>(st, nodeOps, nodeOps, size_t(numNodes) + 1, 0, nullptr, scanWs,
                                               wsBytes - 256))
         return e;
-    int32_t hostVals[2];
-    HC_CHECK(hipMemcpyAsync(&hostVals[0], nodeOps + numNodes, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    HC_CHECK(hipMemcpyAsync(&hostVals[1], flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    // the flag and the new node count side by side, read back with one copy into pinned memory
+    copyI32Kernel<<<1, 1, 0, st>>>(nodeOps + numNodes, reinterpret_cast<int32_t*>(flag) + 1);
+    HC_CHECK_LAUNCH("copyI32Kernel");
+    uint32_t* pin = pinnedStaging();
+    if (!pin) return HIPCSTONE_HIP_ERROR;
+    HC_CHECK(hipMemcpyAsync(pin, flag, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HC_CHECK(hipStreamSynchronize(st));
-    if (newNumNodes) *newNumNodes = hostVals[0];
-    if (converged) *converged = hostVals[1];
+    if (newNumNodes) *newNumNodes = int32_t(pin[1]);
+    if (converged) *converged = int(pin[0]);
     return HIPCSTONE_OK;
 }
 

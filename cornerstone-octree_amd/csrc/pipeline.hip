@@ -217,6 +217,16 @@ public:
     {
     }
 
+    ~Pipeline() override
+    {
+        if (levelRangeReady_)
+        {
+            (void)hipEventSynchronize(levelRangeReady_);
+            (void)hipEventDestroy(levelRangeReady_);
+        }
+        if (levelRangePinned_) (void)hipHostFree(levelRangePinned_);
+    }
+
     hipcstone_status sync(hipcstone_stream s, const void* xv, const void* yv, const void* zv, const void* hv,
                           size_t n, const void* boxv, int fullBuild, uint32_t* nb, uint32_t* nc) override
     {
@@ -256,12 +266,20 @@ public:
         cur_ = out;
         n_   = n;
 
+        // the device's fault word (guards of the previous step's search and of this step's sort) rides along with
+        // the synchronization of the tree build or update
+        uint32_t* pin = pinnedStaging();
+        if (!pin) return HIPCSTONE_HIP_ERROR;
+        uint32_t* faults = pin + 8;
+        *faults          = 0;
+        if (auto e = enqueueFaultRead(stream(s), faults)) return e;
         if (fullBuild || numLeaves_ == 0) { HC_STAGE(2, computeOctree(s)); }
         else
         {
             int conv = 0;
             HC_STAGE(2, updateOnce(s, &conv));
         }
+        if (auto e = checkFaults(*faults)) return e;
         HC_STAGE(3, link(s));
         HC_STAGE(4, A::centers(s, prefixes_.as<K>(), numNodes_, centers_.as<T>(), sizes_.as<T>(), &box_));
         HC_STAGE(5, hipcstone_leaf_layout(s, counts_[treeCur_].as<uint32_t>(), numLeaves_, layout_.as<uint32_t>(),
@@ -365,8 +383,10 @@ public:
         v->centers          = centers_.ptr;
         v->sizes            = sizes_.ptr;
         v->layout           = layout_.ptr;
+        // the level range's copy of the last link (an asynchronous copy into pinned memory: wait for it)
+        if (levelRangeReady_) (void)hipEventSynchronize(levelRangeReady_);
         for (int i = 0; i < 24; ++i)
-            v->level_range_host[i] = i < int(maxLevel<K>() + 2) ? levelRangeHost_[i] : 0;
+            v->level_range_host[i] = levelRangePinned_ && i < int(maxLevel<K>() + 2) ? levelRangePinned_[i] : 0;
         v->converged_calls = convergedCalls_;
         v->assigned_first  = first_;
         v->assigned_last   = last_;
@@ -587,14 +607,22 @@ private:
         if (auto e = layout_.reserve((size_t(L) + 1) * sizeof(uint32_t))) return e;
         if (auto e = scanWs_.reserve(hipcstone_scan_workspace_size(size_t(L)))) return e;
         numNodes_ = N;
-        // the device's fault word rides along with the level-range synchronization of the link
-        uint32_t faults = 0;
-        if (auto e = enqueueFaultRead(stream(s), &faults)) return e;
         if (auto e = A::build(s, tree_[treeCur_].template as<K>(), L, prefixes_.as<K>(), childOffsets_.as<int32_t>(),
                               parents_.as<int32_t>(), levelRange_.as<int32_t>(), internalToLeaf_.as<int32_t>(),
-                              leafToInternal_.as<int32_t>(), buildWs_.ptr, buildWs_.bytes, levelRangeHost_))
+                              leafToInternal_.as<int32_t>(), buildWs_.ptr, buildWs_.bytes, nullptr))
             return e;
-        return checkFaults(faults);
+        // the view's host copy of the level range: an asynchronous copy into pinned memory, waited for by view()
+        if (!levelRangePinned_)
+        {
+            void* q = nullptr;
+            HC_CHECK(hipHostMalloc(&q, 32 * sizeof(int32_t), hipHostMallocDefault));
+            levelRangePinned_ = static_cast<int32_t*>(q);
+            HC_CHECK(hipEventCreateWithFlags(&levelRangeReady_, hipEventDisableTiming));
+        }
+        HC_CHECK(hipMemcpyAsync(levelRangePinned_, levelRange_.ptr, sizeof(int32_t) * (maxLevel<K>() + 2),
+                                hipMemcpyDeviceToHost, stream(s)));
+        HC_CHECK(hipEventRecord(levelRangeReady_, stream(s)));
+        return HIPCSTONE_OK;
     }
 
     int curve_;
@@ -610,7 +638,8 @@ private:
     int cur_           = 0;
     int treeCur_       = 0;
     int convergedCalls_ = 0;
-    int32_t levelRangeHost_[32]{};
+    int32_t* levelRangePinned_  = nullptr; // hipHostMalloc'd
+    hipEvent_t levelRangeReady_ = nullptr;
     int32_t first_     = 0;
     int32_t last_      = 0;
     size_t numLocal_   = 0;

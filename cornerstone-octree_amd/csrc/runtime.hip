@@ -58,6 +58,23 @@ hipcstone_status enqueueFaultRead(hipStream_t s, uint32_t* host)
     return HIPCSTONE_OK;
 }
 
+uint32_t* pinnedStaging()
+{
+    static thread_local uint32_t* p = nullptr;
+    if (!p)
+    {
+        void* q      = nullptr;
+        hipError_t e = hipHostMalloc(&q, kPinnedWords * sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess)
+        {
+            setError(std::string("pinned staging: hipHostMalloc failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        p = static_cast<uint32_t*>(q);
+    }
+    return p;
+}
+
 hipcstone_status checkFaults(uint32_t faults)
 {
     if (faults == 0) return HIPCSTONE_OK;

@@ -35,6 +35,12 @@ uint32_t* faultWord();
 // before a synchronization they make anyway and then call checkFaults().
 hipcstone_status enqueueFaultRead(hipStream_t s, uint32_t* host);
 hipcstone_status checkFaults(uint32_t faults);
+// A small pinned host buffer of the calling thread (kPinnedWords words, allocated on first use with hipHostMalloc and
+// kept until the process ends; nullptr if that failed, the error text is then set). Device-to-host copies into it are
+// asynchronous, so several can ride on one stream synchronization. Words [0, 8) are the tree update's, word 8 the
+// orchestration objects' fault-word copy.
+constexpr int kPinnedWords = 64;
+uint32_t* pinnedStaging();
 // the radix sort's look-back poll limit (hipcstone_set_sort_lookback_limit)
 uint32_t sortLookbackLimit();
 // waves per workgroup of the neighbor search (hipcstone_set_neighbor_waves): 2, 4 or 8

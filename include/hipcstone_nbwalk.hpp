@@ -272,8 +272,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
     uint32_t myWpos = rowOff;
 
     // the wave's output rows as a structured buffer of 4-byte records: a store takes the row slot itself as its
-    // index, and a lane without a hit stores at the record count, past the end, where the hardware's range check
-    // drops it (no branch around the store, no byte-offset arithmetic)
+    // index, and a lane without a hit stores at index 2^32 - 1, past the record count, where the hardware's range
+    // check drops it (no branch around the store, no byte-offset arithmetic)
     const __amdgpu_buffer_rsrc_t rows = __builtin_amdgcn_make_buffer_rsrc(
         rowsBytes ? static_cast<void*>(rowsPtr) : nullptr, 4, int(rowsBytes / 4u), kBufferFlags);
 
@@ -385,14 +385,14 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         if constexpr (decltype(checkSelf)::value) { H &= ballot(jRel != (rec.meta >> 8)); }
                         return H;
                     };
-                    // store index of a lane's pair: its row slot for a hit (inside the row), else the record count,
-                    // past the end of the buffer, where the store is dropped
+                    // store index of a lane's pair: its row slot for a hit (inside the row), else 2^32 - 1, past the
+                    // record count, where the store is dropped (an inline constant: no register holds it)
                     auto storeOffset = [&](const TargetRec<T>& rec, uint64_t H, uint32_t pos) -> uint32_t {
                         if constexpr (decltype(checkLimit)::value)
                         {
-                            return (laneIn(H) & (pos < rec.limit)) ? pos : rowsBytes / 4u;
+                            return (laneIn(H) & (pos < rec.limit)) ? pos : ~0u;
                         }
-                        else { return laneIn(H) ? pos : rowsBytes / 4u; }
+                        else { return laneIn(H) ? pos : ~0u; }
                     };
                     // slot s of an iteration holds targets k0 + s (A) and k0 + S + s (B). In the iterations with
                     // k0 + 2S <= pc every slot holds a target (lanes past the last slot test a NaN particle, which
