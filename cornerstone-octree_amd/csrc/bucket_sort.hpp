@@ -130,40 +130,78 @@ __device__ inline void waveLeafRange(const K* __restrict__ leaves, uint32_t L, K
     b = lo1;
 }
 
+// Leaves the hint looks through: the wave's keys are searched among leaves [hint, hint + kHintLeaves)
+constexpr uint32_t kHintLeaves = 16;
+
+// hint[g] = the leaf of the previous layout that holds position 64 g (the previous step's leaf of the particle that
+// now sits first in wave g): one thread per leaf writes the waves whose first position falls in it. layout[0..L] is
+// the previous leaf layout of exactly n particles.
+template<int Unused = 0>
+__global__ __launch_bounds__(256) void waveLeafHintKernel(const uint32_t* __restrict__ layout, int32_t L, uint32_t n,
+                                                          uint32_t* __restrict__ hint)
+{
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
+    if (l >= uint32_t(L)) return;
+    const uint32_t a = layout[l], b = layout[l + 1];
+    for (uint32_t g = (a + 63u) / 64u; 64u * g < b && 64u * g < n; ++g)
+        hint[g] = l;
+}
+
 // bucket[i] = the leaf of the previous leaf array holding keys[i]; slot[i] = its position among its bucket's keys
-// (any order). segCount[leaf] counts them; it starts at zero.
+// (any order). segCount[leaf] counts them; it starts at zero. hint (optional, from waveLeafHintKernel): when every
+// key of a wave lies in the hint's kHintLeaves leaves, the wave counts boundaries there and skips the search; any
+// other wave searches the whole leaf array (the hint only saves work, it never changes a bucket).
 template<class K>
 __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ keys, uint32_t n,
                                                           const K* __restrict__ leaves, int32_t L,
                                                           uint32_t* __restrict__ segCount, uint32_t* __restrict__ bucket,
-                                                          uint32_t* __restrict__ slot)
+                                                          uint32_t* __restrict__ slot,
+                                                          const uint32_t* __restrict__ hint)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const int lane   = laneId();
     const bool valid = i < n;
     const K key      = valid ? keys[i] : K(0);
     if (ballot(valid) == 0) return; // wave-uniform
-    // the leaf range holding the wave's keys, then a binary search inside it per lane
-    // (keys lie in [leaves[0], leaves[L]) = [0, nodeRange(0)))
-    const K kmin    = waveMinKey(valid ? key : ~K(0));
-    const K kmax    = waveMaxKey(valid ? key : K(0));
-    uint32_t a, b;
-    waveLeafRange(leaves, uint32_t(L), kmin, kmax, lane, a, b);
-    uint32_t lo = a; // leaves[a] <= key < leaves[b + 1]
-    if (b - a <= 16)
+    uint32_t lo = 0;
+    bool found  = false;
+    if (hint)
     {
-        // few leaves: count the boundaries at or below the key (wave-uniform addresses)
-        for (uint32_t j = a + 1; j <= b; ++j)
-            lo += uint32_t(leaves[j] <= key);
-    }
-    else
-    {
-        uint32_t hi = b + 1;
-        while (hi - lo > 1)
+        const uint32_t l0 = uniform(hint[uniform(i / 64u)]);
+        const uint32_t l1 = l0 + kHintLeaves < uint32_t(L) ? l0 + kHintLeaves : uint32_t(L);
+        const bool inside = !valid || (leaves[l0] <= key && key < leaves[l1]);
+        if (ballot(!inside) == 0)
         {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (leaves[mid] <= key) lo = mid;
-            else hi = mid;
+            lo = l0;
+            for (uint32_t j = l0 + 1; j < l1; ++j)
+                lo += uint32_t(leaves[j] <= key);
+            found = true;
+        }
+    }
+    if (!found)
+    {
+        // the leaf range holding the wave's keys, then a binary search inside it per lane
+        // (keys lie in [leaves[0], leaves[L]) = [0, nodeRange(0)))
+        const K kmin = waveMinKey(valid ? key : ~K(0));
+        const K kmax = waveMaxKey(valid ? key : K(0));
+        uint32_t a, b;
+        waveLeafRange(leaves, uint32_t(L), kmin, kmax, lane, a, b);
+        lo = a; // leaves[a] <= key < leaves[b + 1]
+        if (b - a <= 16)
+        {
+            // few leaves: count the boundaries at or below the key (wave-uniform addresses)
+            for (uint32_t j = a + 1; j <= b; ++j)
+                lo += uint32_t(leaves[j] <= key);
+        }
+        else
+        {
+            uint32_t hi = b + 1;
+            while (hi - lo > 1)
+            {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (leaves[mid] <= key) lo = mid;
+                else hi = mid;
+            }
         }
     }
     // one atomic per distinct leaf of the wave: the first lane of each group adds the group's size
@@ -378,11 +416,14 @@ constexpr uint32_t kMaxBucket = 2048;
 // and xo..ho[i] = x..h[perm[i]]; leaves[0..L] is a cornerstone leaf array (leaves[0] = 0, L > 0). Keys at or above
 // leaves[L - 1] go to the last bucket. keyBuf and bucketBuf hold n elements; segWs and scanWs grow as needed. If some
 // bucket holds more than kMaxBucket keys, nothing is written but *done = false (one stream synchronization reads the
-// largest bucket).
+// largest bucket). prevLayout (optional): the leaf layout of these leaves over the previous positions of exactly n
+// particles (keys[i] being the moved particle that sat at position i): it gives each wave of keys a leaf to start its
+// bucket search from (waveLeafHintKernel, into hintWs).
 template<class K, class T>
 hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const K* leaves, int32_t L, const T* x,
                                    const T* y, const T* z, const T* h, T* xo, T* yo, T* zo, T* ho, uint32_t* perm,
-                                   K* keyBuf, uint32_t* bucketBuf, DeviceBuf& segWs, DeviceBuf& scanWs, bool* done)
+                                   K* keyBuf, uint32_t* bucketBuf, DeviceBuf& segWs, DeviceBuf& scanWs, bool* done,
+                                   const uint32_t* prevLayout = nullptr, DeviceBuf* hintWs = nullptr)
 {
     *done = false;
     if (n == 0)
@@ -408,7 +449,17 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
     HC_CHECK_LAUNCH("zeroU32Kernel");
     zeroU32Kernel<<<1, 256, 0, st>>>(maxBucket, 1u);
     HC_CHECK_LAUNCH("zeroU32Kernel");
-    bucketByLeafKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, leaves, L, segCount, bucketBuf, slot);
+    // the previous layout (of the same particle count, in the order the keys come in) gives each wave a leaf to
+    // start from
+    uint32_t* hint = nullptr;
+    if (prevLayout && hintWs)
+    {
+        if (auto e = hintWs->reserve(size_t(ceilDiv(n, 64)) * sizeof(uint32_t))) return e;
+        hint = hintWs->as<uint32_t>();
+        waveLeafHintKernel<<<ceilDiv(size_t(L), 256), 256, 0, st>>>(prevLayout, L, n32, hint);
+        HC_CHECK_LAUNCH("waveLeafHintKernel");
+    }
+    bucketByLeafKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, leaves, L, segCount, bucketBuf, slot, hint);
     HC_CHECK_LAUNCH("bucketByLeafKernel");
     maxBucketKernel<<<std::min(ceilDiv(size_t(L), 256), 1024u), 256, 0, st>>>(segCount, uint32_t(L), maxBucket);
     HC_CHECK_LAUNCH("maxBucketKernel");

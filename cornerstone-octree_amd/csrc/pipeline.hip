@@ -402,7 +402,7 @@ public:
     size_t deviceBytes() const override
     {
         size_t b = 0;
-        for (const DeviceBuf* d : {&keys_, &keyBuf_, &perm_, &valBuf_, &sortWs_, &segWs_, &updWs_, &splitKeys_,
+        for (const DeviceBuf* d : {&keys_, &keyBuf_, &perm_, &valBuf_, &sortWs_, &segWs_, &hintWs_, &updWs_, &splitKeys_,
                                    &splitCounts_, &splitCounters_, &splitSortWs_, &prefixes_, &childOffsets_,
                                    &parents_, &levelRange_, &internalToLeaf_, &leafToInternal_, &buildWs_, &centers_,
                                    &sizes_, &layout_, &scanWs_, &assignOff_, &assignBnd_, &assignCnt_, &assignWs_,
@@ -424,9 +424,12 @@ private:
                                           T* yo, T* zo, T* ho, size_t n)
     {
         bool done = false;
+        // the previous step's layout is over the previous positions of the same particles when their count is
+        // unchanged (a search hint only: any order of the input gives the same result)
+        const uint32_t* prevLayout = n == n_ ? layout_.as<uint32_t>() : nullptr;
         if (auto e = sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, tree_[treeCur_].template as<K>(), numLeaves_, x, y,
                                              z, h, xo, yo, zo, ho, perm_.as<uint32_t>(), keyBuf_.as<K>(),
-                                             valBuf_.as<uint32_t>(), segWs_, scanWs_, &done))
+                                             valBuf_.as<uint32_t>(), segWs_, scanWs_, &done, prevLayout, &hintWs_))
             return e;
         if (done) return HIPCSTONE_OK;
         return A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xo, yo, zo, ho, keyBuf_.as<K>(),
@@ -644,7 +647,7 @@ private:
     int32_t last_      = 0;
     size_t numLocal_   = 0;
 
-    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, segWs_;
+    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, segWs_, hintWs_;
     DeviceBuf xyzh_[2][4];
     DeviceBuf tree_[2], counts_[2], updWs_;
     DeviceBuf frontier_[2], splitKeys_, splitCounts_, splitCounters_, splitSortWs_;
