@@ -265,30 +265,26 @@ __global__ __launch_bounds__(256) void scatterByBucketKernel(const K* __restrict
                                                              const uint32_t* __restrict__ bucket,
                                                              const uint32_t* __restrict__ slot,
                                                              const uint32_t* __restrict__ segOffset,
-                                                             K* __restrict__ keysOut, uint32_t* __restrict__ idxOut,
-                                                             uint2* __restrict__ rangeOut)
+                                                             K* __restrict__ keysOut, uint32_t* __restrict__ idxOut)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    const uint32_t b   = bucket[i];
-    const uint32_t s0  = segOffset[b];
-    const uint32_t pos = s0 + slot[i];
+    const uint32_t pos = segOffset[bucket[i]] + slot[i];
     keysOut[pos]       = keys[i];
     idxOut[pos]        = i;
-    rangeOut[pos]      = make_uint2(s0, segOffset[b + 1]);
 }
 
 // Each wave orders the buckets [g * kBucketsPerWave, (g + 1) * kBucketsPerWave), which are contiguous in the scattered
 // arrays, 64 elements at a time: a window starting at a bucket boundary holds some whole buckets (and the start of
 // the next one, left for the next window); every lane of a whole bucket counts the elements of its bucket that come
 // before it in (key, index) order (the pairs are distinct) and writes its element, with x,y,z,h, to bucket start +
-// rank. A bucket of more than 64 elements is ordered by itself, its elements compared in chunks of 64.
+// rank. A bucket of more than 64 elements is ordered by itself, its elements compared in chunks of 64. Each element's
+// bucket range comes from the wave's 17 bucket offsets, held in scalar registers (no per-element range in memory).
 constexpr uint32_t kBucketsPerWave = 16;
 
 template<class K, class T>
 __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restrict__ keysIn,
                                                                const uint32_t* __restrict__ idxIn,
-                                                               const uint2* __restrict__ rangeIn,
                                                                const uint32_t* __restrict__ segOffset, int32_t L,
                                                                K* __restrict__ keysOut, uint32_t* __restrict__ permOut,
                                                                const T* __restrict__ x, const T* __restrict__ y,
@@ -307,6 +303,11 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
     const uint32_t b0  = uint32_t(g * kBucketsPerWave);
     const uint32_t b1  = uint32_t(std::min<uint64_t>(uint64_t(b0) + kBucketsPerWave, uint64_t(L)));
     const uint32_t end = uniform(segOffset[b1]);
+    // the offsets of the wave's buckets (offset k of bucket b0 + k; past b1 they repeat the end)
+    uint32_t off[kBucketsPerWave + 1];
+#pragma unroll
+    for (uint32_t k = 0; k <= kBucketsPerWave; ++k)
+        off[k] = uniform(segOffset[b0 + k < b1 ? b0 + k : b1]);
     auto put = [&](uint32_t pos, K k, uint32_t id) {
         keysOut[pos] = k;
         permOut[pos] = id;
@@ -322,9 +323,18 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
     auto load = [&](uint32_t p) {
         const uint32_t e = p + uint32_t(lane);
         const bool valid = e < end;
-        r                = valid ? rangeIn[e] : make_uint2(0u, 0u);
-        k                = valid ? keysIn[e] : K(0);
-        id               = valid ? idxIn[e] : 0u;
+        // the bucket [off[j], off[j + 1]) holding e: the last offset at or below e and the first above it (the
+        // offsets ascend; empty buckets repeat an offset)
+        uint32_t rx = off[0], ry = ~0u;
+#pragma unroll
+        for (uint32_t j = 1; j <= kBucketsPerWave; ++j)
+        {
+            rx = off[j] <= e ? off[j] : rx;
+            ry = off[j] > e && off[j] < ry ? off[j] : ry;
+        }
+        r  = valid ? make_uint2(rx, ry) : make_uint2(0u, 0u);
+        k  = valid ? keysIn[e] : K(0);
+        id = valid ? idxIn[e] : 0u;
     };
     uint32_t P = uniform(segOffset[b0]);
     if (P < end) load(P);
@@ -432,18 +442,17 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
         return HIPCSTONE_OK;
     }
     if (L <= 0) return invalid("sortByLeafBuckets: empty leaf array");
-    if (auto e = segWs.reserve((4 * n + 2 * size_t(L) + 4) * sizeof(uint32_t))) return e;
+    if (auto e = segWs.reserve((2 * n + 2 * size_t(L) + 4) * sizeof(uint32_t))) return e;
     if (auto e = scanWs.reserve(hipcstone_scan_workspace_size(size_t(L)))) return e;
     uint32_t* segCount  = segWs.as<uint32_t>();
     uint32_t* segOffset = segCount + L; // L + 1 entries
     uint32_t* slot      = segOffset + L + 1;
     uint32_t* idx       = slot + n;
-    // the ranges start at u32 offset 2L + 2 + 2n (one u32 of padding), 8-byte aligned
-    uint2* range       = reinterpret_cast<uint2*>(idx + n + 1);
+
     hipStream_t st     = stream(s);
     const uint32_t n32 = uint32_t(n);
 
-    uint32_t* maxBucket = reinterpret_cast<uint32_t*>(range + n); // one u32 past the ranges
+    uint32_t* maxBucket = idx + n;
 
     zeroU32Kernel<<<ceilDiv(size_t(L), 256), 256, 0, st>>>(segCount, uint32_t(L));
     HC_CHECK_LAUNCH("zeroU32Kernel");
@@ -468,12 +477,11 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
     HC_CHECK(hipStreamSynchronize(st));
     if (largest > kMaxBucket) return HIPCSTONE_OK;
     if (auto e = hipcstone_leaf_layout(s, segCount, L, segOffset, scanWs.ptr, scanWs.bytes)) return e;
-    scatterByBucketKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, bucketBuf, slot, segOffset, keyBuf, idx,
-                                                               range);
+    scatterByBucketKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, bucketBuf, slot, segOffset, keyBuf, idx);
     HC_CHECK_LAUNCH("scatterByBucketKernel");
     const unsigned blocks = ceilDiv(ceilDiv(size_t(L), kBucketsPerWave), 4);
-    sortBucketsGatherKernel<K, T><<<blocks, 256, 0, st>>>(keyBuf, idx, range, segOffset, L, keys, perm, x, y, z, h,
-                                                          xo, yo, zo, ho);
+    sortBucketsGatherKernel<K, T><<<blocks, 256, 0, st>>>(keyBuf, idx, segOffset, L, keys, perm, x, y, z, h, xo, yo,
+                                                          zo, ho);
     HC_CHECK_LAUNCH("sortBucketsGatherKernel");
     *done = true;
     return HIPCSTONE_OK;
