@@ -428,12 +428,17 @@ constexpr uint32_t kMaxBucket = 2048;
 // bucket holds more than kMaxBucket keys, nothing is written but *done = false (one stream synchronization reads the
 // largest bucket). prevLayout (optional): the leaf layout of these leaves over the previous positions of exactly n
 // particles (keys[i] being the moved particle that sat at position i): it gives each wave of keys a leaf to start its
-// bucket search from (waveLeafHintKernel, into hintWs).
+// bucket search from (waveLeafHintKernel, into hintWs). maxOut (optional, host): receives the largest bucket. With
+// deferMax (maxOut then in pinned memory) it is copied there asynchronously and the bucket path is taken without
+// reading it first (no synchronization): the result is the same for any bucket size (large buckets are ranked in
+// chunks), the guard only bounds the work, so a caller that expects small buckets (they were small in its previous
+// step) reads *maxOut at its next synchronization and guards the step after that.
 template<class K, class T>
 hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const K* leaves, int32_t L, const T* x,
                                    const T* y, const T* z, const T* h, T* xo, T* yo, T* zo, T* ho, uint32_t* perm,
                                    K* keyBuf, uint32_t* bucketBuf, DeviceBuf& segWs, DeviceBuf& scanWs, bool* done,
-                                   const uint32_t* prevLayout = nullptr, DeviceBuf* hintWs = nullptr)
+                                   const uint32_t* prevLayout = nullptr, DeviceBuf* hintWs = nullptr,
+                                   uint32_t* maxOut = nullptr, bool deferMax = false)
 {
     *done = false;
     if (n == 0)
@@ -472,10 +477,15 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
     HC_CHECK_LAUNCH("bucketByLeafKernel");
     maxBucketKernel<<<std::min(ceilDiv(size_t(L), 256), 1024u), 256, 0, st>>>(segCount, uint32_t(L), maxBucket);
     HC_CHECK_LAUNCH("maxBucketKernel");
-    uint32_t largest = 0;
-    HC_CHECK(hipMemcpyAsync(&largest, maxBucket, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HC_CHECK(hipStreamSynchronize(st));
-    if (largest > kMaxBucket) return HIPCSTONE_OK;
+    if (maxOut && deferMax) { HC_CHECK(hipMemcpyAsync(maxOut, maxBucket, sizeof(uint32_t), hipMemcpyDeviceToHost, st)); }
+    else
+    {
+        uint32_t largest = 0;
+        HC_CHECK(hipMemcpyAsync(&largest, maxBucket, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HC_CHECK(hipStreamSynchronize(st));
+        if (maxOut) *maxOut = largest;
+        if (largest > kMaxBucket) return HIPCSTONE_OK;
+    }
     if (auto e = hipcstone_leaf_layout(s, segCount, L, segOffset, scanWs.ptr, scanWs.bytes)) return e;
     scatterByBucketKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, bucketBuf, slot, segOffset, keyBuf, idx);
     HC_CHECK_LAUNCH("scatterByBucketKernel");

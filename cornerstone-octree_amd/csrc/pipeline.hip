@@ -273,13 +273,22 @@ public:
         uint32_t* faults = pin + 8;
         *faults          = 0;
         if (auto e = enqueueFaultRead(stream(s), faults)) return e;
-        if (fullBuild || numLeaves_ == 0) { HC_STAGE(2, computeOctree(s)); }
+        if (fullBuild || numLeaves_ == 0)
+        {
+            HC_STAGE(2, computeOctree(s));
+            lastMaxBucket_ = ~0u; // a new tree: the next warm step reads its largest bucket first
+        }
         else
         {
             int conv = 0;
             HC_STAGE(2, updateOnce(s, &conv));
         }
         if (auto e = checkFaults(*faults)) return e;
+        if (maxPending_)
+        {
+            lastMaxBucket_ = pin[9]; // the largest bucket of this step's sort, copied before the update synchronized
+            maxPending_    = false;
+        }
         HC_STAGE(3, link(s));
         HC_STAGE(4, A::centers(s, prefixes_.as<K>(), numNodes_, centers_.as<T>(), sizes_.as<T>(), &box_));
         HC_STAGE(5, hipcstone_leaf_layout(s, counts_[treeCur_].as<uint32_t>(), numLeaves_, layout_.as<uint32_t>(),
@@ -427,10 +436,22 @@ private:
         // the previous step's layout is over the previous positions of the same particles when their count is
         // unchanged (a search hint only: any order of the input gives the same result)
         const uint32_t* prevLayout = n == n_ ? layout_.as<uint32_t>() : nullptr;
+        // when the previous step's largest bucket was well below the guard, this step's is not read before the
+        // bucket path runs: it comes back with the tree update's synchronization (pinned word 9) and guards the
+        // next step
+        uint32_t* pin = pinnedStaging();
+        if (!pin) return HIPCSTONE_HIP_ERROR;
+        const bool defer = lastMaxBucket_ <= kMaxBucket / 4;
+        uint32_t largest = ~0u;
+        uint32_t* maxOut = defer ? pin + 9 : &largest;
+        *maxOut          = ~0u;
         if (auto e = sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, tree_[treeCur_].template as<K>(), numLeaves_, x, y,
                                              z, h, xo, yo, zo, ho, perm_.as<uint32_t>(), keyBuf_.as<K>(),
-                                             valBuf_.as<uint32_t>(), segWs_, scanWs_, &done, prevLayout, &hintWs_))
+                                             valBuf_.as<uint32_t>(), segWs_, scanWs_, &done, prevLayout, &hintWs_,
+                                             maxOut, defer))
             return e;
+        maxPending_    = defer;
+        lastMaxBucket_ = defer ? ~0u : largest;
         if (done) return HIPCSTONE_OK;
         return A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xo, yo, zo, ho, keyBuf_.as<K>(),
                              valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes);
@@ -641,6 +662,8 @@ private:
     int cur_           = 0;
     int treeCur_       = 0;
     int convergedCalls_ = 0;
+    uint32_t lastMaxBucket_     = ~0u;   // largest bucket of the last warm-step sort, ~0 when unknown
+    bool maxPending_            = false; // this step's largest bucket is on its way to pinned word 9
     int32_t* levelRangePinned_  = nullptr; // hipHostMalloc'd
     hipEvent_t levelRangeReady_ = nullptr;
     int32_t first_     = 0;
