@@ -36,7 +36,8 @@ void findNeighborsKernel(
     const int32_t* __restrict__ internalToLeaf, const uint32_t* __restrict__ layout, const T* __restrict__ centers,
     const T* __restrict__ sizes, int32_t numNodes, float ext, uint32_t ngmax, uint32_t* __restrict__ neighbors,
     uint32_t* __restrict__ neighborsCount, uint32_t numBlocksPadded, uint32_t* __restrict__ faults,
-    const uint32_t* __restrict__ waves = nullptr, uint32_t numWaves = 0, const int32_t* __restrict__ roots = nullptr)
+    const uint32_t* __restrict__ waves = nullptr, uint32_t numWaves = 0, const int32_t* __restrict__ roots = nullptr,
+    const nbwalk::ChildGroup* __restrict__ groups = nullptr)
 {
     __shared__ nbwalk::Shared<T, kNbWaves> sh;
     const int lane       = laneId();
@@ -70,7 +71,7 @@ void findNeighborsKernel(
     const uint32_t validRows = uint64_t(lastId) - first < uint64_t(kWave) ? uint32_t(uint64_t(lastId) - first) : kWave;
     const uint32_t rowsBytes = ngmax ? (SlotMajor ? uint32_t(kWave) : validRows) * ngmax * uint32_t(sizeof(uint32_t))
                                      : 0u;
-    const nbwalk::Tree<T> tree{childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext};
+    const nbwalk::Tree<T> tree{childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext, groups};
     const uint32_t count = nbwalk::walk<T, AnyPbc, SlotMajor, Segmented, kNbWaves>(
         x, y, z, h, box, tree, root, first, valid, ngmax, ngmax ? neighbors + rowBase * ngmax : nullptr, rowsBytes,
         sh, w, faults);
@@ -114,13 +115,73 @@ template hipcstone_status findNeighborsSegmented<float>(hipStream_t, const float
                                                         int32_t, uint32_t, uint32_t*, uint32_t*, const uint32_t*,
                                                         uint32_t, const int32_t*);
 
+// the compact child blocks of an fp32 tree (nbwalk::ChildGroup): one thread per block of 8 children [8 g + 1, 8 g + 9)
+__global__ void childGroupsKernel(const float* __restrict__ centers, const float* __restrict__ sizes,
+                                  const int32_t* __restrict__ childOffsets, const int32_t* __restrict__ internalToLeaf,
+                                  uint32_t numGroups, nbwalk::ChildGroup* __restrict__ groups)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= numGroups) return;
+    const size_t co = size_t(g) * 8 + 1;
+    nbwalk::ChildGroup r;
+    bool same      = true;
+    uint32_t perm  = 0;
+    uint32_t inner = 0;
+    for (int d = 0; d < 3; ++d)
+    {
+        const uint32_t a = __float_as_uint(centers[co * 3 + d]);
+        const uint32_t s = __float_as_uint(sizes[co * 3 + d]);
+        uint32_t b       = a;
+        for (int k = 1; k < 8; ++k)
+        {
+            const uint32_t c = __float_as_uint(centers[(co + k) * 3 + d]);
+            if (c != a)
+            {
+                if (b == a) b = c;
+                same = same && c == b;
+                perm |= 1u << (4 * k + 2 - d);
+            }
+            same = same && __float_as_uint(sizes[(co + k) * 3 + d]) == s;
+        }
+        r.v0[d] = __uint_as_float(a);
+        r.v1[d] = __uint_as_float(b);
+        r.s[d]  = __uint_as_float(s);
+    }
+    for (int k = 0; k < 8; ++k)
+    {
+        inner |= uint32_t(childOffsets[co + k] != 0) << k;
+        r.leaf[k] = internalToLeaf[co + k];
+    }
+    r.perm    = perm;
+    r.flags   = inner | (same ? nbwalk::kGroupValid : 0u);
+    groups[g] = r;
+}
+
+size_t neighborGroupsBytes(int32_t numNodes)
+{
+    return numNodes > 8 ? size_t((numNodes - 1) / 8) * sizeof(nbwalk::ChildGroup) : 0;
+}
+
+hipcstone_status buildChildGroups(hipStream_t s, const float* centers, const float* sizes,
+                                  const int32_t* childOffsets, const int32_t* internalToLeaf, int32_t numNodes,
+                                  nbwalk::ChildGroup* groups)
+{
+    const uint32_t numGroups = numNodes > 8 ? uint32_t((numNodes - 1) / 8) : 0u;
+    if (numGroups == 0) return HIPCSTONE_OK;
+    childGroupsKernel<<<(numGroups + 255) / 256, 256, 0, s>>>(centers, sizes, childOffsets, internalToLeaf,
+                                                                        numGroups, groups);
+    HC_CHECK_LAUNCH("childGroupsKernel");
+    return HIPCSTONE_OK;
+}
+
 template<class T>
 static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h,
                                       uint32_t firstId, uint32_t lastId, const typename BoxOf<T>::type* box,
                                       const int32_t* childOffsets, const int32_t* parents,
                                       const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,
                                       const T* sizes, int32_t numNodes, float ext, uint32_t ngmax,
-                                      uint32_t* neighbors, uint32_t* neighborsCount, bool slotMajor)
+                                      uint32_t* neighbors, uint32_t* neighborsCount, bool slotMajor,
+                                      void* groupsWs = nullptr, size_t groupsWsBytes = 0)
 {
     if (lastId <= firstId) return HIPCSTONE_OK;
     if (!x || !y || !z || !h || !box || !childOffsets || !parents || !internalToLeaf || !layout || !centers ||
@@ -129,19 +190,44 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
     if (ngmax > 0 && !neighbors) return invalid("findNeighbors: neighbors is null with ngmax > 0");
     if (ngmax > (1u << 20)) return invalid("findNeighbors: ngmax above 2^20");
     if (numNodes <= 0) return invalid("findNeighbors: numNodes must be positive");
-    size_t groups   = (size_t(lastId - firstId) + kWave - 1) / kWave;
+    size_t waveGroups = (size_t(lastId - firstId) + kWave - 1) / kWave;
     uint32_t* faults = faultWord();
     if (!faults) return HIPCSTONE_HIP_ERROR;
     bool anyPbc     = box->bnd[0] == HIPCSTONE_PERIODIC || box->bnd[1] == HIPCSTONE_PERIODIC ||
                   box->bnd[2] == HIPCSTONE_PERIODIC;
+    // fp32 open boxes walk through the compact child blocks, built here into the caller's workspace (or into memory
+    // allocated on the stream for this call)
+    void* owned                       = nullptr;
+    const nbwalk::ChildGroup* groups = nullptr;
+    if constexpr (std::is_same_v<T, float>)
+    {
+        const size_t need = neighborGroupsBytes(numNodes);
+        if (!anyPbc && need > 0)
+        {
+            void* dst = groupsWs;
+            if (!dst || groupsWsBytes < need)
+            {
+                HC_CHECK(hipMallocAsync(&owned, need, stream(s)));
+                dst = owned;
+            }
+            if (auto e = buildChildGroups(stream(s), centers, sizes, childOffsets, internalToLeaf, numNodes,
+                                          static_cast<nbwalk::ChildGroup*>(dst)))
+            {
+                if (owned) (void)hipFreeAsync(owned, stream(s));
+                return e;
+            }
+            groups = static_cast<const nbwalk::ChildGroup*>(dst);
+        }
+    }
     auto shaped = [&](auto pbc, auto slots, auto wv) {
         constexpr int W = decltype(wv)::value;
-        unsigned blocks = ceilDiv(groups, W);
+        unsigned blocks = ceilDiv(waveGroups, W);
         blocks          = (blocks + kXcds - 1) / kXcds * kXcds; // whole rounds over the XCDs; extra groups exit
         findNeighborsKernel<T, decltype(pbc)::value, decltype(slots)::value, false, W>
             <<<blocks, W * kWave, 0, stream(s)>>>(x, y, z, h, firstId, lastId, *box, childOffsets, internalToLeaf,
                                                   layout, centers, sizes, numNodes, ext, ngmax, neighbors,
-                                                  neighborsCount, blocks, faults);
+                                                  neighborsCount, blocks, faults, nullptr, 0u, nullptr,
+                                                  decltype(pbc)::value ? nullptr : groups);
     };
     auto launch = [&](auto pbc, auto slots) {
         switch (neighborWaves())
@@ -155,9 +241,31 @@ static hipcstone_status findNeighbors(hipcstone_stream s, const T* x, const T* y
     else if (anyPbc) launch(std::true_type{}, std::false_type{});
     else if (slotMajor) launch(std::false_type{}, std::true_type{});
     else launch(std::false_type{}, std::false_type{});
-    HC_CHECK_LAUNCH("findNeighborsKernel");
+    hipError_t launched = hipGetLastError();
+    if (owned) HC_CHECK(hipFreeAsync(owned, stream(s)));
+    if (launched != hipSuccess) return hipFail(launched, "findNeighborsKernel");
     return HIPCSTONE_OK;
 }
+
+template<class T>
+hipcstone_status findNeighborsWs(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, uint32_t firstId,
+                                 uint32_t lastId, const typename BoxOf<T>::type* box, const int32_t* childOffsets,
+                                 const int32_t* parents, const int32_t* internalToLeaf, const uint32_t* layout,
+                                 const T* centers, const T* sizes, int32_t numNodes, float ext, uint32_t ngmax,
+                                 uint32_t* neighbors, uint32_t* neighborsCount, void* groupsWs, size_t groupsWsBytes)
+{
+    return findNeighbors<T>(s, x, y, z, h, firstId, lastId, box, childOffsets, parents, internalToLeaf, layout,
+                            centers, sizes, numNodes, ext, ngmax, neighbors, neighborsCount, false, groupsWs,
+                            groupsWsBytes);
+}
+
+#define HIPCSTONE_INST_NEIGHBORS_WS(T)                                                                                 \
+    template hipcstone_status findNeighborsWs<T>(                                                                      \
+        hipcstone_stream, const T*, const T*, const T*, const T*, uint32_t, uint32_t, const BoxOf<T>::type*,           \
+        const int32_t*, const int32_t*, const int32_t*, const uint32_t*, const T*, const T*, int32_t, float, uint32_t, \
+        uint32_t*, uint32_t*, void*, size_t);
+HIPCSTONE_INST_NEIGHBORS_WS(float)
+HIPCSTONE_INST_NEIGHBORS_WS(double)
 
 } // namespace hipcstone
 

@@ -108,10 +108,11 @@ struct Api;
         static hipcstone_status neighbors(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h,         \
                                           uint32_t f, uint32_t l, const Box* b, const int32_t* co,                    \
                                           const int32_t* par, const int32_t* i2l, const uint32_t* lay, const T* c,     \
-                                          const T* sz, int32_t N, uint32_t ngmax, uint32_t* nb, uint32_t* nc)          \
+                                          const T* sz, int32_t N, uint32_t ngmax, uint32_t* nb, uint32_t* nc,          \
+                                          void* gws, size_t gwsb)                                                      \
         {                                                                                                              \
-            return hipcstone_find_neighbors_##RS(s, x, y, z, h, f, l, b, co, par, i2l, lay, c, sz, N, 1.0f, ngmax, nb,\
-                                                 nc);                                                                  \
+            return findNeighborsWs<T>(s, x, y, z, h, f, l, b, co, par, i2l, lay, c, sz, N, 1.0f, ngmax, nb, nc, gws,  \
+                                      gwsb);                                                                           \
         }                                                                                                              \
     };
 
@@ -295,9 +296,11 @@ public:
                                           scanWs_.ptr, scanWs_.bytes));
         if (nc)
         {
+            if (auto e = groupsWs_.reserve(neighborGroupsBytes(numNodes_))) return e;
             HC_STAGE(6, A::neighbors(s, xo, yo, zo, ho, 0, uint32_t(n), &box_, childOffsets_.as<int32_t>(),
                                      parents_.as<int32_t>(), internalToLeaf_.as<int32_t>(), layout_.as<uint32_t>(),
-                                     centers_.as<T>(), sizes_.as<T>(), numNodes_, nb ? ngmax_ : 0u, nb, nc));
+                                     centers_.as<T>(), sizes_.as<T>(), numNodes_, nb ? ngmax_ : 0u, nb, nc,
+                                     groupsWs_.ptr, groupsWs_.bytes));
         }
 #undef HC_STAGE
         return HIPCSTONE_OK;
@@ -415,7 +418,7 @@ public:
                                    &splitCounts_, &splitCounters_, &splitSortWs_, &prefixes_, &childOffsets_,
                                    &parents_, &levelRange_, &internalToLeaf_, &leafToInternal_, &buildWs_, &centers_,
                                    &sizes_, &layout_, &scanWs_, &assignOff_, &assignBnd_, &assignCnt_, &assignWs_,
-                                   &flags_, &searchC_, &searchS_, &localLayout_, &haloWs_})
+                                   &flags_, &searchC_, &searchS_, &localLayout_, &haloWs_, &groupsWs_})
             b += d->bytes;
         for (int t = 0; t < 2; ++t)
         {
@@ -675,7 +678,7 @@ private:
     DeviceBuf tree_[2], counts_[2], updWs_;
     DeviceBuf frontier_[2], splitKeys_, splitCounts_, splitCounters_, splitSortWs_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
-    DeviceBuf centers_, sizes_, layout_, scanWs_;
+    DeviceBuf centers_, sizes_, layout_, scanWs_, groupsWs_;
     DeviceBuf assignOff_, assignBnd_, assignCnt_, assignWs_, flags_, searchC_, searchS_, localLayout_, haloWs_;
 };
 

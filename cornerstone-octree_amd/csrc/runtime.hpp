@@ -46,6 +46,18 @@ uint32_t sortLookbackLimit();
 // waves per workgroup of the neighbor search (hipcstone_set_neighbor_waves): 2, 4 or 8
 int neighborWaves();
 
+// The neighbor search of hipcstone_find_neighbors_* with a workspace for the compact child blocks the fp32 open-box
+// walk reads (neighborGroupsBytes(numNodes) bytes; with less, or none, they are allocated on the stream per call).
+size_t neighborGroupsBytes(int32_t numNodes);
+template<class T>
+struct BoxOf;
+template<class T>
+hipcstone_status findNeighborsWs(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, uint32_t firstId,
+                                 uint32_t lastId, const typename BoxOf<T>::type* box, const int32_t* childOffsets,
+                                 const int32_t* parents, const int32_t* internalToLeaf, const uint32_t* layout,
+                                 const T* centers, const T* sizes, int32_t numNodes, float ext, uint32_t ngmax,
+                                 uint32_t* neighbors, uint32_t* neighborsCount, void* groupsWs, size_t groupsWsBytes);
+
 } // namespace hipcstone
 
 // Return the status of a failed HIP call from the enclosing C ABI function.

@@ -141,6 +141,36 @@ __device__ inline uint32_t addLaneBit(uint32_t v, uint64_t mask)
     return r;
 }
 
+// all ones if bit b of the wave-uniform v is set, else zero (one scalar sign-extending bit-field extract)
+__device__ inline uint32_t sbfeBit(uint32_t v, int b) { return uint32_t(__builtin_amdgcn_sbfe(int(v), b, 1)); }
+
+// a1 where the wave-uniform m is set, a0 elsewhere, as one v_bfi_b32 (the compiler would otherwise turn the bit
+// select of an all-ones-or-zero mask into a scalar compare, a scalar select of a lane mask and a v_cndmask)
+__device__ inline float pickBits(uint32_t m, uint32_t a1, uint32_t a0)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(m), "v"(a1), "v"(a0));
+    return __uint_as_float(r);
+}
+
+// bit k set iff m[k] != 0, two scalar instructions per mask (compare, then shift the compare bit in)
+__device__ inline uint32_t nonzeroBits8(const uint64_t* m)
+{
+    uint32_t acc = 0;
+    asm("s_cmp_lg_u64 %1, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %2, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %3, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %4, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %5, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %6, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %7, 0\n\ts_addc_u32 %0, %0, %0\n\t"
+        "s_cmp_lg_u64 %8, 0\n\ts_addc_u32 %0, %0, %0"
+        : "+s"(acc)
+        : "s"(m[7]), "s"(m[6]), "s"(m[5]), "s"(m[4]), "s"(m[3]), "s"(m[2]), "s"(m[1]), "s"(m[0])
+        : "scc");
+    return acc;
+}
+
 // the structured buffer store of the LLVM AMDGPU back end (address = base + vindex * stride; with stride != 0 a
 // vindex at or past the record count is out of range and the store is dropped), which has no clang builtin
 __device__ void structBufferStoreB32(uint32_t data, __amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
@@ -181,6 +211,26 @@ struct Shared
     TargetRec<T> ctgt[kNbWaves * kWave + kWave];
 };
 
+// The 8 children of one internal node in a compact form (fp32 trees): in each dimension the children's centers take
+// two values and their sizes one (true of geometric centers, which depend only on a node's integer box), so a
+// child's distance terms are one of two values per dimension, computed once per lane for all 8 children. Record g
+// describes the children [8 g + 1, 8 g + 9), the child block of some internal node in the level-major layout.
+//   v0, v1, s: the two center values and the size per dimension (v1 == v0 where only one value occurs)
+//   perm:      bit 4 k + 2 - d is set if child k's center in dimension d is v1
+//   flags:     bit k set if child k is internal; kGroupValid if every child matched v0/v1 and s bit for bit (the
+//              walk falls back to the children's own centers and sizes otherwise)
+//   leaf:      the children's leaf indices (internalToLeaf)
+struct alignas(128) ChildGroup
+{
+    float v0[3], v1[3], s[3];
+    uint32_t perm;
+    uint32_t flags;
+    uint32_t pad[5];
+    int32_t leaf[8];
+};
+typedef uint32_t ChildGroupHead __attribute__((ext_vector_type(16))); // dwords 0-15 of a ChildGroup, one load
+constexpr uint32_t kGroupValid = 0x80000000u;
+
 // a linked octree with its geometry and leaf layout (device pointers; centers and sizes as x,y,z triples). A kernel
 // that walks should take these arrays (and x, y, z, h) as separate __restrict__ arguments and assemble the Tree
 // itself: only then does the compiler know they are read-only and unaliased and fetch the wave-uniform node values
@@ -195,6 +245,8 @@ struct Tree
     const T* sizes;
     int32_t numNodes;
     float ext;
+    // optional (fp32, open boxes, one tree): the child blocks in compact form, numbered by (first child - 1) / 8
+    const ChildGroup* groups = nullptr;
 };
 
 /*
@@ -221,6 +273,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
     const T* __restrict__ centers              = tree.centers;
     const T* __restrict__ sizes                = tree.sizes;
     const int32_t numNodes                     = tree.numNodes;
+    constexpr bool kGroups = std::is_same_v<T, float> && !AnyPbc && !Segmented;
+    const ChildGroup* __restrict__ groups = kGroups ? tree.groups : nullptr;
 
     const int lane   = laneId();
     const uint32_t i = uint32_t(first) + uint32_t(lane);
@@ -471,7 +525,67 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
     // the bottom of the tree); otherwise the children's lane masks are pushed as the frame at `depth`. The 48 node
     // values and the 16 child offsets / leaf indices are loaded up front (contiguous, wave-uniform) and the tests
     // are branch-free. Returns whether a frame was pushed.
+    auto pushFrame = [&](int depth, int32_t co, const uint64_t* mk) {
+        if (lane == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+            {
+                sh.mask[w][depth][k] = mk[k];
+            }
+            sh.co[w][depth]   = co;
+            sh.next[w][depth] = 0;
+        }
+        waveSync();
+    };
     auto descend = [&](int depth, int32_t co, uint64_t pm) -> bool {
+        if constexpr (kGroups)
+        {
+            // compact child block: per lane two distance terms per dimension, each child's terms picked with a
+            // bit-field select (v_bfi with a wave-uniform mask); the sums are the per-child nodeDist2 bit for bit
+            if (groups != nullptr && ((co - 1) & 7) == 0)
+            {
+                const ChildGroup* __restrict__ g = groups + ((co - 1) >> 3);
+                // geometry, perm and flags arrive with one 64-byte scalar load
+                const ChildGroupHead hd = *reinterpret_cast<const ChildGroupHead*>(g);
+                const uint32_t flags    = hd[10];
+                if (flags & kGroupValid)
+                {
+                    const uint32_t perm = hd[9];
+                    auto term = [&](uint32_t c, uint32_t s, float p) -> uint32_t {
+                        float d = fmaxf(fabsf(__uint_as_float(c) - p) - __uint_as_float(s), 0.0f);
+                        return __float_as_uint(d * d);
+                    };
+                    const uint32_t ax0 = term(hd[0], hd[6], xi);
+                    const uint32_t ax1 = term(hd[3], hd[6], xi);
+                    const uint32_t ay0 = term(hd[1], hd[7], yi);
+                    const uint32_t ay1 = term(hd[4], hd[7], yi);
+                    const uint32_t az0 = term(hd[2], hd[8], zi);
+                    const uint32_t az1 = term(hd[5], hd[8], zi);
+                    uint64_t mk[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        const float dx = pickBits(sbfeBit(perm, 4 * k + 2), ax1, ax0);
+                        const float dy = pickBits(sbfeBit(perm, 4 * k + 1), ay1, ay0);
+                        const float dz = pickBits(sbfeBit(perm, 4 * k), az1, az0);
+                        mk[k]          = ballot(dx + (dy + dz) < cellR2) & pm;
+                    }
+                    const uint32_t passed = nonzeroBits8(mk);
+                    if ((passed & flags & 0xFFu) == 0)
+                    {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            if (passed & (1u << k)) appendLeaf(g->leaf[k], mk[k]);
+                        }
+                        return false;
+                    }
+                    pushFrame(depth, co, mk);
+                    return true;
+                }
+            }
+        }
         const T* c = centers + size_t(co) * 3;
         const T* s = sizes + size_t(co) * 3;
         T cv[24], sv[24];
@@ -512,17 +626,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
             }
             return false;
         }
-        if (lane == 0)
-        {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-            {
-                sh.mask[w][depth][k] = mk[k];
-            }
-            sh.co[w][depth]   = co;
-            sh.next[w][depth] = 0;
-        }
-        waveSync();
+        pushFrame(depth, co, mk);
         return true;
     };
 

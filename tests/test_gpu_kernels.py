@@ -275,6 +275,39 @@ def test_find_neighbors(hip, oracle, T, kd):
         assert neighbor_rows_equal(nb, nc, ref_nb, ref_nc, ngmax)
 
 
+def test_find_neighbors_mixed_child_geometry(hip, oracle):
+    """fp32 open-box searches read each block of 8 children in a compact form (two center values and one size per
+    dimension) and fall back to the children's own centers and sizes for blocks where that form does not hold. Here
+    some blocks are made irregular by one-ulp changes of a child's center or size, so both forms run in one walk;
+    rows must still equal the oracle's on the same (changed) geometry, with and without a search extension."""
+    T = np.float32
+    n = 50000
+    x, y, z, h = plummer_input(n, T, a=0.2, target=80)
+    gbox, obox = boxes(hip, T, -1.0, 1.0)
+    p = cpu_pipeline(oracle, x, y, z, h, obox, np.uint64, 32, HILBERT)
+    N = p["prefixes"].size
+    centers = p["centers"].reshape(-1, 3).copy()
+    sizes = p["sizes"].reshape(-1, 3).copy()
+    rng = np.random.default_rng(7)
+    blocks = (N - 1) // 8
+    assert blocks > 50
+    picked = rng.choice(blocks, size=blocks // 5, replace=False)
+    for i, b in enumerate(picked):
+        node = 1 + 8 * int(b) + int(rng.integers(0, 8))
+        dim = int(rng.integers(0, 3))
+        if i % 2 == 0:
+            centers[node, dim] = np.nextafter(centers[node, dim], T(np.inf))
+        else:
+            sizes[node, dim] = np.nextafter(sizes[node, dim], T(np.inf))
+    q = dict(p, centers=centers.reshape(-1), sizes=sizes.reshape(-1))
+    tree = {k: p[k] for k in ("childOffsets", "parents", "internalToLeaf")}
+    for ext in (1.0, 1.2):
+        ref_nb, ref_nc = oracle.find_neighbors(p["x"], p["y"], p["z"], p["h"], 0, n, obox, tree, p["layout"],
+                                               q["centers"], q["sizes"], 96, ext)
+        nb, nc = _gpu_neighbors(hip, q, gbox, 0, n, 96, ext)
+        assert neighbor_rows_equal(nb, nc, ref_nb, ref_nc, 96), ext
+
+
 @pytest.mark.parametrize("boundary", [(1, 1, 1), (1, 0, 0)])
 def test_find_neighbors_pbc_ext_subrange(hip, oracle, boundary):
     T = np.float64
