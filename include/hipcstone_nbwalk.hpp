@@ -453,8 +453,9 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                     // fails every compare), so no lane mask applies; the last, partial iteration masks the slots
                     // past pc with wave-uniform masks, and has no B half if pc - k0 <= S. Both records are read
                     // before either count is written back.
-                    auto body2 = [&](TargetRec<T>* rA, uint64_t inA, uint64_t inB, uint64_t wA,
-                                     uint64_t wB) __attribute__((always_inline)) {
+                    // sameWriteBack: wA == wB (full iterations), both counts are written back under one lane mask
+                    auto body2 = [&](TargetRec<T>* rA, uint64_t inA, uint64_t inB, uint64_t wA, uint64_t wB,
+                                     auto sameWriteBack) __attribute__((always_inline)) {
                         TargetRec<T>* rB        = rA + S;
                         const TargetRec<T> recA = *rA;
                         const TargetRec<T> recB = *rB;
@@ -464,8 +465,19 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         const uint32_t posB     = position(HB, recB.wpos);
                         structBufferStoreB32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0, 0);
                         structBufferStoreB32(j, rows, int(storeOffset(recB, HB, posB)), 0, 0, 0);
-                        if (laneIn(wA)) rA->wpos = advance(posA, HA);
-                        if (laneIn(wB)) rB->wpos = advance(posB, HB);
+                        if constexpr (decltype(sameWriteBack)::value)
+                        {
+                            if (laneIn(wA))
+                            {
+                                rA->wpos = advance(posA, HA);
+                                rB->wpos = advance(posB, HB);
+                            }
+                        }
+                        else
+                        {
+                            if (laneIn(wA)) rA->wpos = advance(posA, HA);
+                            if (laneIn(wB)) rB->wpos = advance(posB, HB);
+                        }
                     };
                     auto body1 = [&](TargetRec<T>* rA, uint64_t inA, uint64_t wA) __attribute__((always_inline)) {
                         const TargetRec<T> recA = *rA;
@@ -478,12 +490,12 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                     uint32_t left    = pc;
                     for (; left >= 2 * S; left -= 2 * S, rA += 2 * S)
                     {
-                        body2(rA, ~uint64_t(0), ~uint64_t(0), lastLanes, lastLanes);
+                        body2(rA, ~uint64_t(0), ~uint64_t(0), lastLanes, lastLanes, std::true_type{});
                     }
                     if (left > S)
                     {
                         const uint64_t inB = slotsBelow(left - S);
-                        body2(rA, ~uint64_t(0), inB, lastLanes, lastLanes & inB);
+                        body2(rA, ~uint64_t(0), inB, lastLanes, lastLanes & inB, std::false_type{});
                     }
                     else if (left > 0)
                     {
