@@ -115,60 +115,15 @@ template hipcstone_status findNeighborsSegmented<float>(hipStream_t, const float
                                                         int32_t, uint32_t, uint32_t*, uint32_t*, const uint32_t*,
                                                         uint32_t, const int32_t*);
 
-// the compact child blocks of an fp32 tree (nbwalk::ChildGroup): one thread per block of 8 children [8 g + 1, 8 g + 9)
-__global__ void childGroupsKernel(const float* __restrict__ centers, const float* __restrict__ sizes,
-                                  const int32_t* __restrict__ childOffsets, const int32_t* __restrict__ internalToLeaf,
-                                  uint32_t numGroups, nbwalk::ChildGroup* __restrict__ groups)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= numGroups) return;
-    const size_t co = size_t(g) * 8 + 1;
-    nbwalk::ChildGroup r;
-    bool same      = true;
-    uint32_t perm  = 0;
-    uint32_t inner = 0;
-    for (int d = 0; d < 3; ++d)
-    {
-        const uint32_t a = __float_as_uint(centers[co * 3 + d]);
-        const uint32_t s = __float_as_uint(sizes[co * 3 + d]);
-        uint32_t b       = a;
-        for (int k = 1; k < 8; ++k)
-        {
-            const uint32_t c = __float_as_uint(centers[(co + k) * 3 + d]);
-            if (c != a)
-            {
-                if (b == a) b = c;
-                same = same && c == b;
-                perm |= 1u << (4 * k + 2 - d);
-            }
-            same = same && __float_as_uint(sizes[(co + k) * 3 + d]) == s;
-        }
-        r.v0[d] = __uint_as_float(a);
-        r.v1[d] = __uint_as_float(b);
-        r.s[d]  = __uint_as_float(s);
-    }
-    for (int k = 0; k < 8; ++k)
-    {
-        inner |= uint32_t(childOffsets[co + k] != 0) << k;
-        r.leaf[k] = internalToLeaf[co + k];
-    }
-    r.perm    = perm;
-    r.flags   = inner | (same ? nbwalk::kGroupValid : 0u);
-    groups[g] = r;
-}
-
-size_t neighborGroupsBytes(int32_t numNodes)
-{
-    return numNodes > 8 ? size_t((numNodes - 1) / 8) * sizeof(nbwalk::ChildGroup) : 0;
-}
+size_t neighborGroupsBytes(int32_t numNodes) { return nbwalk::childGroupCount(numNodes) * sizeof(nbwalk::ChildGroup); }
 
 hipcstone_status buildChildGroups(hipStream_t s, const float* centers, const float* sizes,
                                   const int32_t* childOffsets, const int32_t* internalToLeaf, int32_t numNodes,
                                   nbwalk::ChildGroup* groups)
 {
-    const uint32_t numGroups = numNodes > 8 ? uint32_t((numNodes - 1) / 8) : 0u;
+    const uint32_t numGroups = uint32_t(nbwalk::childGroupCount(numNodes));
     if (numGroups == 0) return HIPCSTONE_OK;
-    childGroupsKernel<<<(numGroups + 255) / 256, 256, 0, s>>>(centers, sizes, childOffsets, internalToLeaf,
+    nbwalk::childGroupsKernel<<<(numGroups + 255) / 256, 256, 0, s>>>(centers, sizes, childOffsets, internalToLeaf,
                                                                         numGroups, groups);
     HC_CHECK_LAUNCH("childGroupsKernel");
     return HIPCSTONE_OK;

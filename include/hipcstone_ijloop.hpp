@@ -314,12 +314,12 @@ __global__ __launch_bounds__(kFusedWaves * 64) void fusedIjLoopKernel(
     const T* __restrict__ h, const Record<T>* __restrict__ rec, InTuple input, OutTuple output,
     Interaction interaction, Postamble postamble, unsigned ngmax, LocalIndex first, LocalIndex last,
     GroupView groups, LocalIndex* __restrict__ scratch, uint32_t* __restrict__ counters,
-    uint32_t* __restrict__ faults)
+    uint32_t* __restrict__ faults, const nbwalk::ChildGroup* __restrict__ childGroups)
 {
     using namespace hipcstone;
     // the tree's arrays come in as separate restrict-qualified arguments: read-only memory that nothing aliases,
     // so the walk's wave-uniform node reads become scalar loads
-    const nbwalk::Tree<T> tree{childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext};
+    const nbwalk::Tree<T> tree{childOffsets, internalToLeaf, layout, centers, sizes, numNodes, ext, childGroups};
     constexpr auto inIdx = std::make_index_sequence<std::tuple_size_v<InTuple>>{};
     __shared__ nbwalk::Shared<T, kFusedWaves> sh;
     const int w             = uniform(int(threadIdx.x) / kWave);
@@ -456,7 +456,7 @@ public:
     Statistics stats() const
     {
         return {std::size_t(groups_.lastBody - groups_.firstBody),
-                scratchBytes_ + recordBytes_ + hipcstone::kXcds * sizeof(uint32_t)};
+                scratchBytes_ + recordBytes_ + childGroupBytes_ + hipcstone::kXcds * sizeof(uint32_t)};
     }
 
     struct Subgroup
@@ -537,11 +537,30 @@ private:
         detail::checkHip(hipGetLastError(), "ijloop: record staging");
         detail::zeroCountersKernel<<<1, 64, 0, stream_>>>(counters_.get());
         detail::checkHip(hipGetLastError(), "ijloop: group counters");
+        // fp32 open boxes: the compact child blocks the walk reads (built once per neighborhood, the tree is fixed)
+        const nbwalk::ChildGroup* childGroups = nullptr;
+        if constexpr (std::is_same_v<T, float> && !AnyPbc)
+        {
+            const std::size_t count = nbwalk::childGroupCount(tree_.numNodes);
+            if (count > 0 && !childGroups_)
+            {
+                void* p = nullptr;
+                detail::checkHip(hipMalloc(&p, count * sizeof(nbwalk::ChildGroup)), "ijloop: child blocks");
+                childGroups_.reset(p);
+                childGroupBytes_ = count * sizeof(nbwalk::ChildGroup);
+                nbwalk::childGroupsKernel<><<<unsigned((count + 255) / 256), 256, 0, stream_>>>(
+                    reinterpret_cast<const float*>(tree_.centers), reinterpret_cast<const float*>(tree_.sizes),
+                    tree_.childOffsets, tree_.internalToLeaf, uint32_t(count),
+                    static_cast<nbwalk::ChildGroup*>(childGroups_.get()));
+                detail::checkHip(hipGetLastError(), "ijloop: child blocks");
+            }
+            childGroups = static_cast<const nbwalk::ChildGroup*>(childGroups_.get());
+        }
         kernel<<<unsigned(blocks), kThreads, 0, stream_>>>(
             box_, tree_.childOffsets, tree_.internalToLeaf, tree_.layout, reinterpret_cast<const T*>(tree_.centers),
             reinterpret_cast<const T*>(tree_.sizes), tree_.numNodes, tree_.searchExtFactor, x_, y_, z_, h_,
             rec, input, output, interaction, postamble, ngmax_, first, last, groups, scratch_.get(),
-            counters_.get(), faults_);
+            counters_.get(), faults_, childGroups);
         detail::checkHip(hipGetLastError(), "ijloop: fused search and interaction kernel");
     }
 
@@ -558,6 +577,8 @@ private:
     std::unique_ptr<uint32_t, detail::DeviceFree> counters_;
     mutable std::size_t recordBytes_ = 0;
     mutable std::unique_ptr<void, detail::DeviceFree> records_;
+    mutable std::size_t childGroupBytes_ = 0;
+    mutable std::unique_ptr<void, detail::DeviceFree> childGroups_;
 };
 
 // gpu_alwaystraverse.cuh:180-206

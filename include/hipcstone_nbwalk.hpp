@@ -231,6 +231,53 @@ struct alignas(128) ChildGroup
 typedef uint32_t ChildGroupHead __attribute__((ext_vector_type(16))); // dwords 0-15 of a ChildGroup, one load
 constexpr uint32_t kGroupValid = 0x80000000u;
 
+// the ChildGroup records of an fp32 tree with numNodes nodes: (numNodes - 1) / 8 of them (none for a root-only tree)
+inline size_t childGroupCount(int32_t numNodes) { return numNodes > 8 ? size_t((numNodes - 1) / 8) : 0; }
+
+// builds the ChildGroup records, one thread per block of 8 children [8 g + 1, 8 g + 9); a template so that every
+// translation unit that launches it has its own copy
+template<int = 0>
+__global__ void childGroupsKernel(const float* __restrict__ centers, const float* __restrict__ sizes,
+                                  const int32_t* __restrict__ childOffsets, const int32_t* __restrict__ internalToLeaf,
+                                  uint32_t numGroups, ChildGroup* __restrict__ groups)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= numGroups) return;
+    const size_t co = size_t(g) * 8 + 1;
+    ChildGroup r{};
+    bool same      = true;
+    uint32_t perm  = 0;
+    uint32_t inner = 0;
+    for (int d = 0; d < 3; ++d)
+    {
+        const uint32_t a = __float_as_uint(centers[co * 3 + d]);
+        const uint32_t s = __float_as_uint(sizes[co * 3 + d]);
+        uint32_t b       = a;
+        for (int k = 1; k < 8; ++k)
+        {
+            const uint32_t c = __float_as_uint(centers[(co + k) * 3 + d]);
+            if (c != a)
+            {
+                if (b == a) b = c;
+                same = same && c == b;
+                perm |= 1u << (4 * k + 2 - d);
+            }
+            same = same && __float_as_uint(sizes[(co + k) * 3 + d]) == s;
+        }
+        r.v0[d] = __uint_as_float(a);
+        r.v1[d] = __uint_as_float(b);
+        r.s[d]  = __uint_as_float(s);
+    }
+    for (int k = 0; k < 8; ++k)
+    {
+        inner |= uint32_t(childOffsets[co + k] != 0) << k;
+        r.leaf[k] = internalToLeaf[co + k];
+    }
+    r.perm    = perm;
+    r.flags   = inner | (same ? kGroupValid : 0u);
+    groups[g] = r;
+}
+
 // a linked octree with its geometry and leaf layout (device pointers; centers and sizes as x,y,z triples). A kernel
 // that walks should take these arrays (and x, y, z, h) as separate __restrict__ arguments and assemble the Tree
 // itself: only then does the compiler know they are read-only and unaliased and fetch the wave-uniform node values
