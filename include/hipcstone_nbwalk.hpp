@@ -110,12 +110,18 @@ namespace nbwalk
 constexpr int kMaxDepth = 22; // 21 levels below the root for 64-bit keys
 constexpr int kLeafList = 64;
 
-// per leaf-chunk size nj in [1, 64]: slots = 64 / nj and the multiplier with floor(l / nj) == (l * magic) >> 16
-// for every lane l < 64 (checked exhaustively), so the per-chunk slot layout needs no division
+// per leaf-chunk size nj in [1, 64]: slots = 64 / nj, the multiplier with floor(l / nj) == (l * magic) >> 16 for
+// every lane l < 64 (checked exhaustively), so the per-chunk slot layout needs no division, and the lane mask of the
+// slots' last lanes {nj - 1, 2 nj - 1, ..., slots * nj - 1}; one 16-byte scalar load per chunk
+struct SlotInfo
+{
+    uint32_t magic;
+    uint32_t slots;
+    uint64_t last;
+};
 struct SlotTables
 {
-    uint32_t magic[kWave + 1];
-    uint32_t slots[kWave + 1];
+    SlotInfo e[kWave + 1];
 };
 
 constexpr SlotTables makeSlotTables()
@@ -123,8 +129,11 @@ constexpr SlotTables makeSlotTables()
     SlotTables t{};
     for (int nj = 1; nj <= kWave; ++nj)
     {
-        t.magic[nj] = (65536u + uint32_t(nj) - 1) / uint32_t(nj);
-        t.slots[nj] = uint32_t(kWave / nj);
+        t.e[nj].magic = (65536u + uint32_t(nj) - 1) / uint32_t(nj);
+        t.e[nj].slots = uint32_t(kWave / nj);
+        t.e[nj].last  = 0;
+        for (int k = 1; k <= kWave / nj; ++k)
+            t.e[nj].last |= uint64_t(1) << (k * nj - 1);
     }
     return t;
 }
@@ -176,6 +185,10 @@ __device__ inline uint32_t nonzeroBits8(const uint64_t* m)
 __device__ void structBufferStoreB32(uint32_t data, __amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
                                      int aux) __asm("llvm.amdgcn.struct.ptr.buffer.store.i32");
 
+// the matching structured buffer load of one 32-bit value (an index at or past the record count reads zero)
+__device__ float structBufferLoadF32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                     int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+
 // dword 3 of an untyped, unswizzled buffer resource on gfx9-family targets (32-bit data format); with a stride in
 // dword 1, accesses are range checked against the record count in dword 2
 constexpr int kBufferFlags = 0x00020000;
@@ -202,7 +215,7 @@ struct Shared
 {
     uint64_t mask[kNbWaves][kMaxDepth][8];
     int32_t co[kNbWaves][kMaxDepth];
-    int32_t next[kNbWaves][kMaxDepth];
+    uint32_t rem[kNbWaves][kMaxDepth]; // bit k: child k of the frame has lanes and is not taken yet
     int32_t listLeaf[kNbWaves][kLeafList];
     uint64_t listMask[kNbWaves][kLeafList];
     // the current leaf's active targets, compacted in lane order: wave w's records are [w * 64, w * 64 + 64); the
@@ -372,6 +385,16 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                              uint32_t((px ? 1 : 0) | (py ? 2 : 0) | (pz ? 4 : 0)) | (uint32_t(lane) << 8), 0u};
     uint32_t myWpos = rowOff;
 
+    // fp32 particle coordinates as structured buffers of 4-byte records (no range limit): the leaf phase's loads
+    // then take the particle index itself, with no 64-bit address arithmetic
+    [[maybe_unused]] __amdgpu_buffer_rsrc_t xRes, yRes, zRes;
+    if constexpr (std::is_same_v<T, float>)
+    {
+        xRes = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(x), 4, -1, kBufferFlags);
+        yRes = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(y), 4, -1, kBufferFlags);
+        zRes = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(z), 4, -1, kBufferFlags);
+    }
+
     // the wave's output rows as a structured buffer of 4-byte records: a store takes the row slot itself as its
     // index, and a lane without a hit stores at index 2^32 - 1, past the record count, where the hardware's range
     // check drops it (no branch around the store, no byte-offset arithmetic)
@@ -414,19 +437,18 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
             waveSync();
             for (uint32_t cs = jb, nj = 0; cs < je; cs += nj)
             {
-                nj             = je - cs < 32u ? je - cs : 32u;
-                uint32_t magic = uniform(kSlotTables.magic[nj]);
-                uint32_t S     = uniform(kSlotTables.slots[nj]);
-                uint32_t slot  = __umul24(uint32_t(lane), magic) >> 16; // lane < 64, magic <= 2^16
-                uint32_t jj    = uint32_t(lane) - slot * nj;
-                bool jv        = slot < S;
-                uint32_t j     = cs + jj;
+                nj                = je - cs < 32u ? je - cs : 32u;
+                const SlotInfo si = kSlotTables.e[nj];
+                const uint32_t S  = si.slots;
+                uint32_t slot     = __umul24(uint32_t(lane), si.magic) >> 16; // lane < 64, magic <= 2^16
+                uint32_t jj       = uint32_t(lane) - slot * nj;
+                uint32_t j        = cs + jj;
                 // lanes of this and later slots (slot * nj < 64): a hit's rank among its slot's hits
-                const uint64_t notBelow = ~((uint64_t(1) << (slot * nj)) - 1);
+                const uint64_t notBelow = ~uint64_t(0) << (slot * nj);
                 const uint32_t nbLo     = uint32_t(notBelow);
                 const uint32_t nbHi     = uint32_t(notBelow >> 32);
                 // the slots' last lanes: each advances the count of its slot's target
-                const uint64_t lastLanes = ballot(jv && jj == nj - 1);
+                const uint64_t lastLanes = si.last;
                 // lanes of the first b slots, [0, b * nj), as a wave-uniform mask (scalar arithmetic)
                 auto slotsBelow = [&](uint32_t b) -> uint64_t {
                     const uint32_t bits = b * nj;
@@ -434,11 +456,24 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                 };
                 // lanes past the last slot test a NaN particle, which fails every distance compare
                 T xj = std::numeric_limits<T>::quiet_NaN(), yj = xj, zj = xj;
-                if (jv)
+                if constexpr (std::is_same_v<T, float>)
                 {
-                    xj = x[j];
-                    yj = y[j];
-                    zj = z[j];
+                    // every lane loads (spare lanes a particle of the chunk), only x is replaced by the NaN: no
+                    // lane mask around the loads
+                    const uint32_t jc = j < je ? j : je - 1;
+                    xj                = structBufferLoadF32(xRes, int(jc), 0, 0, 0);
+                    yj                = structBufferLoadF32(yRes, int(jc), 0, 0, 0);
+                    zj                = structBufferLoadF32(zRes, int(jc), 0, 0, 0);
+                    xj                = laneIn(slotsBelow(S)) ? xj : std::numeric_limits<T>::quiet_NaN();
+                }
+                else
+                {
+                    if (slot < S)
+                    {
+                        xj = x[j];
+                        yj = y[j];
+                        zj = z[j];
+                    }
                 }
                 // j == i is possible only if the chunk overlaps this wave's own targets
                 const bool selfChunk = cs < uint32_t(first) + uint32_t(kWave) && cs + nj > uint32_t(first);
@@ -495,6 +530,20 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         }
                         else { return laneIn(H) ? pos : ~0u; }
                     };
+                    // the hits' stores: with the row-end check, every lane stores (a miss or a slot past the row end
+                    // at an index past the record count); without it, only the hit lanes store, under their lane
+                    // mask (no per-lane index select)
+                    auto storeHits = [&](const TargetRec<T>& rec, uint64_t H, uint32_t pos)
+                                         __attribute__((always_inline)) {
+                        if constexpr (decltype(checkLimit)::value)
+                        {
+                            structBufferStoreB32(j, rows, int(storeOffset(rec, H, pos)), 0, 0, 0);
+                        }
+                        else
+                        {
+                            if (laneIn(H)) structBufferStoreB32(j, rows, int(pos), 0, 0, 0);
+                        }
+                    };
                     // slot s of an iteration holds targets k0 + s (A) and k0 + S + s (B). In the iterations with
                     // k0 + 2S <= pc every slot holds a target (lanes past the last slot test a NaN particle, which
                     // fails every compare), so no lane mask applies; the last, partial iteration masks the slots
@@ -510,8 +559,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         const uint64_t HB       = hits(recB, inB);
                         const uint32_t posA     = position(HA, recA.wpos);
                         const uint32_t posB     = position(HB, recB.wpos);
-                        structBufferStoreB32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0, 0);
-                        structBufferStoreB32(j, rows, int(storeOffset(recB, HB, posB)), 0, 0, 0);
+                        storeHits(recA, HA, posA);
+                        storeHits(recB, HB, posB);
                         if constexpr (decltype(sameWriteBack)::value)
                         {
                             if (laneIn(wA))
@@ -530,7 +579,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         const TargetRec<T> recA = *rA;
                         const uint64_t HA       = hits(recA, inA);
                         const uint32_t posA     = position(HA, recA.wpos);
-                        structBufferStoreB32(j, rows, int(storeOffset(recA, HA, posA)), 0, 0, 0);
+                        storeHits(recA, HA, posA);
                         if (laneIn(wA)) rA->wpos = advance(posA, HA);
                     };
                     TargetRec<T>* rA = recs + slot; // index < 128: pc <= 64 and slot <= S <= 64
@@ -584,7 +633,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
     // the bottom of the tree); otherwise the children's lane masks are pushed as the frame at `depth`. The 48 node
     // values and the 16 child offsets / leaf indices are loaded up front (contiguous, wave-uniform) and the tests
     // are branch-free. Returns whether a frame was pushed.
-    auto pushFrame = [&](int depth, int32_t co, const uint64_t* mk) {
+    auto pushFrame = [&](int depth, int32_t co, const uint64_t* mk, uint32_t passed) {
         if (lane == 0)
         {
 #pragma unroll
@@ -592,8 +641,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
             {
                 sh.mask[w][depth][k] = mk[k];
             }
-            sh.co[w][depth]   = co;
-            sh.next[w][depth] = 0;
+            sh.co[w][depth]  = co;
+            sh.rem[w][depth] = passed;
         }
         waveSync();
     };
@@ -640,7 +689,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                         }
                         return false;
                     }
-                    pushFrame(depth, co, mk);
+                    pushFrame(depth, co, mk, passed);
                     return true;
                 }
             }
@@ -685,7 +734,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
             }
             return false;
         }
-        pushFrame(depth, co, mk);
+        pushFrame(depth, co, mk, nonzeroBits8(mk));
         return true;
     };
 
@@ -704,30 +753,28 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
         else
         {
             int depth         = descend(0, rootChild, rootMask) ? 0 : -1;
-            int64_t stepsLeft = 2 * int64_t(numNodes) + 8;
+            // a 32-bit budget (saturated), so that the per-step test is a scalar compare
+            const uint64_t budget = 2 * uint64_t(uint32_t(numNodes)) + 8;
+            uint32_t stepsLeft    = budget > 0xFFFFFFFFu ? 0xFFFFFFFFu : uint32_t(budget);
             while (depth >= 0)
             {
-                if (stepsLeft-- <= 0)
+                if (stepsLeft-- == 0)
                 {
                     fault = HIPCSTONE_FAULT_WALK_BUDGET;
                     break;
                 }
                 if (listCount > kLeafList - 9) drain();
-                int k      = uniform(sh.next[w][depth]);
-                uint64_t m = 0;
-                while (k < 8)
-                {
-                    m = uniform64(sh.mask[w][depth][k]);
-                    if (m != 0) break;
-                    ++k;
-                }
-                if (k >= 8)
+                // the frame's next child with lanes: the lowest bit of its remaining-children word
+                const uint32_t rem = uniform(sh.rem[w][depth]);
+                if (rem == 0)
                 {
                     --depth;
                     continue;
                 }
-                int32_t node = uniform(sh.co[w][depth]) + k;
-                if (lane == 0) sh.next[w][depth] = k + 1;
+                const int k        = __builtin_ctz(rem);
+                const uint64_t m   = uniform64(sh.mask[w][depth][k]);
+                int32_t node       = uniform(sh.co[w][depth]) + k;
+                if (lane == 0) sh.rem[w][depth] = rem & (rem - 1);
                 waveSync();
                 if (node <= 0 || node >= numNodes)
                 {
