@@ -50,7 +50,7 @@ hipcstone_status findNeighborsSegmented(hipStream_t s, const T* x, const T* y, c
                                         const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,
                                         const T* sizes, int32_t numNodesCap, uint32_t ngmax, uint32_t* neighbors,
                                         uint32_t* neighborsCount, const uint32_t* waves, uint32_t numWaves,
-                                        const int32_t* roots);
+                                        const int32_t* roots, const nbwalk::ChildGroup* groups);
 
 namespace batched
 {
@@ -701,6 +701,9 @@ private:
             if (auto e = reserve(*b, N * sizeof(int32_t))) return e;
         for (auto* b : {&centers_, &sizes_})
             if (auto e = reserve(*b, N * 3 * sizeof(T))) return e;
+        if (auto e = reserve(groups_, std::max<size_t>(nbwalk::childGroupCount(int32_t(N)), 1) *
+                                          sizeof(nbwalk::ChildGroup)))
+            return e;
         capL_ = cap;
         capN_ = int32_t(N);
         return HIPCSTONE_OK;
@@ -849,6 +852,16 @@ private:
         if (auto e = scan::exclusiveScan<uint32_t>(st, counts_.as<uint32_t>(), layout_.as<uint32_t>(), size_t(capL_),
                                                    0u, layout_.as<uint32_t>() + capL_, scanWs_.ptr, scanWs_.bytes))
             return e;
+        // the compact child blocks of all clouds' trees (child blocks start at 1 + 8 s): blocks past the live
+        // node count are built from stale arrays and never read
+        const uint32_t numGroups = uint32_t(nbwalk::childGroupCount(capN_));
+        if (numGroups > 0)
+        {
+            nbwalk::childGroupsKernel<><<<ceilDiv(size_t(numGroups), 256), 256, 0, st>>>(
+                centers_.as<T>(), sizes_.as<T>(), childOffsets_.as<int32_t>(), internalToLeaf_.as<int32_t>(),
+                numGroups, groups_.as<nbwalk::ChildGroup>());
+            HC_CHECK_LAUNCH("batched childGroupsKernel");
+        }
         Box any = hostBoxes_[0];
         for (int d = 0; d < 3; ++d)
             any.bnd[d] = HIPCSTONE_OPEN;
@@ -856,7 +869,8 @@ private:
                                          xyzh_[cur_][3].as<T>(), any, childOffsets_.as<int32_t>(),
                                          internalToLeaf_.as<int32_t>(), layout_.as<uint32_t>(), centers_.as<T>(),
                                          sizes_.as<T>(), capN_, ngmax_, ngmax_ ? nb_.as<uint32_t>() : nullptr,
-                                         nc_.as<uint32_t>(), wavesD_.as<uint32_t>(), numWaves_, roots_.as<int32_t>());
+                                         nc_.as<uint32_t>(), wavesD_.as<uint32_t>(), numWaves_, roots_.as<int32_t>(),
+                                         numGroups ? groups_.as<nbwalk::ChildGroup>() : nullptr);
     }
 
     static hipcstone_status growKeep(DeviceBuf& b, size_t bytes, size_t keep, hipStream_t st)
@@ -890,7 +904,7 @@ private:
     DeviceBuf xyzh_[2][4];
     DeviceBuf offsetsD_, bucketD_, boxesD_, wavesD_, roots_, dL_, metaD_, splitCounters_;
     DeviceBuf tree_[2], counts_, ops_, layout_, scanWs_, opsScanWs_;
-    DeviceBuf prefixes_, childOffsets_, internalToLeaf_, leafToInternal_, parents_, centers_, sizes_;
+    DeviceBuf prefixes_, childOffsets_, internalToLeaf_, leafToInternal_, parents_, centers_, sizes_, groups_;
     DeviceBuf frontier_[2], splitKeys_, splitCounts_, leafSortWs_;
 };
 

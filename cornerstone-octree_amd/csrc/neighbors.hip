@@ -85,7 +85,7 @@ hipcstone_status findNeighborsSegmented(hipStream_t s, const T* x, const T* y, c
                                         const int32_t* internalToLeaf, const uint32_t* layout, const T* centers,
                                         const T* sizes, int32_t numNodesCap, uint32_t ngmax, uint32_t* neighbors,
                                         uint32_t* neighborsCount, const uint32_t* waves, uint32_t numWaves,
-                                        const int32_t* roots)
+                                        const int32_t* roots, const nbwalk::ChildGroup* groups)
 {
     if (numWaves == 0) return HIPCSTONE_OK;
     if (ngmax > (1u << 20)) return invalid("findNeighbors: ngmax above 2^20");
@@ -97,7 +97,7 @@ hipcstone_status findNeighborsSegmented(hipStream_t s, const T* x, const T* y, c
         blocks          = (blocks + kXcds - 1) / kXcds * kXcds;
         findNeighborsKernel<T, false, false, true, W><<<blocks, W * kWave, 0, s>>>(
             x, y, z, h, 0u, 0u, box, childOffsets, internalToLeaf, layout, centers, sizes, numNodesCap, 1.0f, ngmax,
-            neighbors, neighborsCount, blocks, faults, waves, numWaves, roots);
+            neighbors, neighborsCount, blocks, faults, waves, numWaves, roots, groups);
     };
     switch (neighborWaves())
     {
@@ -113,7 +113,7 @@ template hipcstone_status findNeighborsSegmented<float>(hipStream_t, const float
                                                         const float*, const BoxOf<float>::type&, const int32_t*,
                                                         const int32_t*, const uint32_t*, const float*, const float*,
                                                         int32_t, uint32_t, uint32_t*, uint32_t*, const uint32_t*,
-                                                        uint32_t, const int32_t*);
+                                                        uint32_t, const int32_t*, const nbwalk::ChildGroup*);
 
 size_t neighborGroupsBytes(int32_t numNodes) { return nbwalk::childGroupCount(numNodes) * sizeof(nbwalk::ChildGroup); }
 

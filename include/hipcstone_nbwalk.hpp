@@ -305,7 +305,7 @@ struct Tree
     const T* sizes;
     int32_t numNodes;
     float ext;
-    // optional (fp32, open boxes, one tree): the child blocks in compact form, numbered by (first child - 1) / 8
+    // optional (fp32, open boxes): the child blocks in compact form, numbered by (first child - 1) / 8
     const ChildGroup* groups = nullptr;
 };
 
@@ -333,7 +333,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
     const T* __restrict__ centers              = tree.centers;
     const T* __restrict__ sizes                = tree.sizes;
     const int32_t numNodes                     = tree.numNodes;
-    constexpr bool kGroups = std::is_same_v<T, float> && !AnyPbc && !Segmented;
+    constexpr bool kGroups = std::is_same_v<T, float> && !AnyPbc;
     const ChildGroup* __restrict__ groups = kGroups ? tree.groups : nullptr;
 
     const int lane   = laneId();
