@@ -110,36 +110,46 @@ namespace nbwalk
 constexpr int kMaxDepth = 22; // 21 levels below the root for 64-bit keys
 constexpr int kLeafList = 64;
 
-// per leaf-chunk size nj in [1, 64]: slots = 64 / nj, the multiplier with floor(l / nj) == (l * magic) >> 16 for
-// every lane l < 64 (checked exhaustively), so the per-chunk slot layout needs no division, and the lane mask of the
-// slots' last lanes {nj - 1, 2 nj - 1, ..., slots * nj - 1}; one 16-byte scalar load per chunk
+// per leaf-chunk size nj in [1, 32]: slots = 64 / nj, the multiplier with floor(l / nj) == (l * magic) >> 16 for
+// every lane l < 64 (checked exhaustively), so the per-chunk slot layout needs no division, the lane mask of the
+// slots' last lanes {nj - 1, 2 nj - 1, ..., slots * nj - 1} and that of all slot lanes [0, slots * nj); one 32-byte
+// scalar load per chunk
 struct SlotInfo
 {
     uint32_t magic;
     uint32_t slots;
     uint64_t last;
+    uint64_t inSlots;
+    uint64_t pad;
 };
-struct SlotTables
+constexpr int kMaxChunk = 32;
+struct SlotTable
 {
-    SlotInfo e[kWave + 1];
+    SlotInfo e[kMaxChunk + 1];
 };
 
-constexpr SlotTables makeSlotTables()
+constexpr SlotTable makeSlotTable()
 {
-    SlotTables t{};
-    for (int nj = 1; nj <= kWave; ++nj)
+    SlotTable t{};
+    for (int nj = 1; nj <= kMaxChunk; ++nj)
     {
-        t.e[nj].magic = (65536u + uint32_t(nj) - 1) / uint32_t(nj);
-        t.e[nj].slots = uint32_t(kWave / nj);
-        t.e[nj].last  = 0;
-        for (int k = 1; k <= kWave / nj; ++k)
+        const int slots = kWave / nj;
+        t.e[nj].magic   = (65536u + uint32_t(nj) - 1) / uint32_t(nj);
+        t.e[nj].slots   = uint32_t(slots);
+        t.e[nj].last    = 0;
+        for (int k = 1; k <= slots; ++k)
             t.e[nj].last |= uint64_t(1) << (k * nj - 1);
+        t.e[nj].inSlots = slots * nj >= kWave ? ~uint64_t(0) : (uint64_t(1) << (slots * nj)) - 1;
     }
     return t;
 }
 
-// one copy per translation unit (each has its own code object)
-static __constant__ SlotTables kSlotTables = makeSlotTables();
+// one copy per translation unit, internal to it, so that device code addresses it relative to the program counter
+// (no global offset table entry to load first)
+namespace
+{
+constexpr SlotTable kSlotTable = makeSlotTable();
+}
 
 // v + (this lane's bit of the wave-uniform mask) as one add with the mask as carry-in
 __device__ inline uint32_t addLaneBit(uint32_t v, uint64_t mask)
@@ -437,8 +447,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
             waveSync();
             for (uint32_t cs = jb, nj = 0; cs < je; cs += nj)
             {
-                nj                = je - cs < 32u ? je - cs : 32u;
-                const SlotInfo si = kSlotTables.e[nj];
+                nj                = je - cs < uint32_t(kMaxChunk) ? je - cs : uint32_t(kMaxChunk);
+                const SlotInfo si = kSlotTable.e[nj];
                 const uint32_t S  = si.slots;
                 uint32_t slot     = __umul24(uint32_t(lane), si.magic) >> 16; // lane < 64, magic <= 2^16
                 uint32_t jj       = uint32_t(lane) - slot * nj;
@@ -464,7 +474,7 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                     xj                = structBufferLoadF32(xRes, int(jc), 0, 0, 0);
                     yj                = structBufferLoadF32(yRes, int(jc), 0, 0, 0);
                     zj                = structBufferLoadF32(zRes, int(jc), 0, 0, 0);
-                    xj                = laneIn(slotsBelow(S)) ? xj : std::numeric_limits<T>::quiet_NaN();
+                    xj                = laneIn(si.inSlots) ? xj : std::numeric_limits<T>::quiet_NaN();
                 }
                 else
                 {
@@ -476,7 +486,8 @@ walk(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, 
                     }
                 }
                 // j == i is possible only if the chunk overlaps this wave's own targets
-                const bool selfChunk = cs < uint32_t(first) + uint32_t(kWave) && cs + nj > uint32_t(first);
+                // (one unsigned compare: cs + nj - 1 - first wraps to a large value when the chunk ends before first)
+                const bool selfChunk = cs + nj - 1u - uint32_t(first) <= uint32_t(kWave) - 2u + nj;
                 const uint32_t jRel  = j - uint32_t(first);
 
                 // the pair test of particle j against a target record: distance (and, in chunks that overlap the
