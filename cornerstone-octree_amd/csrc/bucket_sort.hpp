@@ -161,21 +161,34 @@ __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ 
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const int lane   = laneId();
     const bool valid = i < n;
-    const K key      = valid ? keys[i] : K(0);
     if (ballot(valid) == 0) return; // wave-uniform
+    // every lane loads (a clamped index past the end), so the key and the hint are in flight together
+    const K key = keys[valid ? i : n - 1];
     uint32_t lo = 0;
     bool found  = false;
     if (hint)
     {
         const uint32_t l0 = uniform(hint[uniform(i / 64u)]);
-        const uint32_t l1 = l0 + kHintLeaves < uint32_t(L) ? l0 + kHintLeaves : uint32_t(L);
-        const bool inside = !valid || (leaves[l0] <= key && key < leaves[l1]);
-        if (ballot(!inside) == 0)
+        if (l0 + kHintLeaves <= uint32_t(L))
         {
-            lo = l0;
-            for (uint32_t j = l0 + 1; j < l1; ++j)
-                lo += uint32_t(leaves[j] <= key);
-            found = true;
+            // the hint's kHintLeaves + 1 boundaries with wave-uniform addresses: a few wide scalar loads issued
+            // together instead of one dependent load per boundary
+            const K* __restrict__ window = leaves + l0;
+            K lv[kHintLeaves + 1];
+#pragma unroll
+            for (uint32_t t = 0; t <= kHintLeaves; ++t)
+                lv[t] = window[t];
+            // counted before the window test, so that all the window's loads are issued before any is waited for
+            uint32_t c = l0;
+#pragma unroll
+            for (uint32_t t = 1; t < kHintLeaves; ++t)
+                c += uint32_t(lv[t] <= key);
+            const bool inside = !valid || (lv[0] <= key && key < lv[kHintLeaves]);
+            if (ballot(!inside) == 0)
+            {
+                lo    = c;
+                found = true;
+            }
         }
     }
     if (!found)
@@ -295,7 +308,7 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
     __shared__ K shKey[256];
     __shared__ uint32_t shIdx[256];
     const int lane = laneId();
-    const int w    = threadIdx.x / kWave;
+    const int w    = uniform(int(threadIdx.x) / kWave); // wave-uniform, so the offsets below are scalar loads
     K* wKey        = shKey + w * kWave;
     uint32_t* wIdx = shIdx + w * kWave;
     const uint64_t g = uint64_t(blockIdx.x) * 4 + uint64_t(w);
@@ -303,11 +316,22 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
     const uint32_t b0  = uint32_t(g * kBucketsPerWave);
     const uint32_t b1  = uint32_t(std::min<uint64_t>(uint64_t(b0) + kBucketsPerWave, uint64_t(L)));
     const uint32_t end = uniform(segOffset[b1]);
-    // the offsets of the wave's buckets (offset k of bucket b0 + k; past b1 they repeat the end)
+    // the offsets of the wave's buckets (offset k of bucket b0 + k; past b1 they repeat the end): for all but the
+    // last wave, one window of kBucketsPerWave + 1 offsets (a few wide scalar loads)
     uint32_t off[kBucketsPerWave + 1];
+    if (b1 == b0 + kBucketsPerWave)
+    {
+        const uint32_t* __restrict__ window = segOffset + b0;
 #pragma unroll
-    for (uint32_t k = 0; k <= kBucketsPerWave; ++k)
-        off[k] = uniform(segOffset[b0 + k < b1 ? b0 + k : b1]);
+        for (uint32_t k = 0; k <= kBucketsPerWave; ++k)
+            off[k] = window[k];
+    }
+    else
+    {
+#pragma unroll
+        for (uint32_t k = 0; k <= kBucketsPerWave; ++k)
+            off[k] = uniform(segOffset[b0 + k < b1 ? b0 + k : b1]);
+    }
     auto put = [&](uint32_t pos, K k, uint32_t id) {
         keysOut[pos] = k;
         permOut[pos] = id;
