@@ -185,6 +185,45 @@ static hipcstone_status rebalanceTree(hipcstone_stream s, const K* tree, int32_t
     return HIPCSTONE_OK;
 }
 
+// Warm-step node counts from the particle positions of the old leaf boundaries: oldPos[i] = lower_bound(keys,
+// tree[i]) for the old leaf array (the bucket offsets of a sort through those leaves, i in [0, numNodes]). Every
+// boundary of the rebalanced array is an old boundary (kept or merged nodes, the first child of a split), whose
+// position is oldPos, or a split child's start inside old node i, whose lower bound lies in [oldPos[i], oldPos[i+1])
+// and is searched there only. One thread per old node writes the positions of the boundaries it produced (the same
+// mapping as rebalanceKernel); counts are then differences of consecutive positions, exactly computeNodeCounts'.
+template<class K>
+__global__ __launch_bounds__(256) void warmBoundaryPosKernel(const K* __restrict__ tree, int32_t numNodes,
+                                                             const int32_t* __restrict__ nodeOps,
+                                                             const K* __restrict__ newTree,
+                                                             const K* __restrict__ keys,
+                                                             const uint32_t* __restrict__ oldPos,
+                                                             uint32_t* __restrict__ pos)
+{
+    int32_t i = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == numNodes) pos[nodeOps[numNodes]] = oldPos[numNodes];
+    if (i >= numNodes) return;
+    const int32_t op  = nodeOps[i + 1] - nodeOps[i];
+    const int32_t dst = nodeOps[i];
+    if (op == 0) return;
+    const uint32_t a = oldPos[i];
+    pos[dst]         = a;
+    if (op >= 8)
+    {
+        const uint32_t b = oldPos[i + 1];
+        for (int32_t c = 1; c < op; ++c)
+            pos[dst + c] = a + uint32_t(lowerBound(keys + a, size_t(b - a), newTree[dst + c]));
+    }
+}
+
+__global__ __launch_bounds__(256) void countsFromPosKernel(const uint32_t* __restrict__ pos, int32_t numNodes,
+                                                           uint32_t maxCount, uint32_t* __restrict__ counts)
+{
+    int32_t i = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= numNodes) return;
+    const uint32_t c = pos[i + 1] - pos[i];
+    counts[i]        = c < maxCount ? c : maxCount;
+}
+
 // workspace: [nodeOps (numNodes+1) int32][node-ops workspace]
 static size_t updateWorkspace(int32_t numNodes)
 {
@@ -195,7 +234,8 @@ template<class K>
 static hipcstone_status updateOctree(hipcstone_stream s, const K* keys, size_t numKeys, uint32_t bucketSize,
                                      const K* tree, const uint32_t* counts, int32_t* numNodes, K* newTree,
                                      uint32_t* newCounts, int32_t capacity, uint32_t maxCount, void* ws,
-                                     size_t wsBytes, int* converged)
+                                     size_t wsBytes, int* converged, const uint32_t* oldPos = nullptr,
+                                     uint32_t* posWs = nullptr)
 {
     if (!numNodes || *numNodes <= 0) return invalid("updateOctree: numNodes must be positive");
     if (bucketSize == 0) return invalid("updateOctree: bucketSize must be positive");
@@ -216,10 +256,35 @@ static hipcstone_status updateOctree(hipcstone_stream s, const K* keys, size_t n
         return HIPCSTONE_CAPACITY;
     }
     if (auto e = rebalanceTree<K>(s, tree, n, newN, nodeOps, newTree)) return e;
-    if (auto e = computeNodeCounts<K>(s, newTree, newCounts, newN, keys, numKeys, maxCount)) return e;
+    if (oldPos && posWs)
+    {
+        warmBoundaryPosKernel<K><<<ceilDiv(size_t(n) + 1, 256), 256, 0, stream(s)>>>(tree, n, nodeOps, newTree, keys,
+                                                                                      oldPos, posWs);
+        HC_CHECK_LAUNCH("warmBoundaryPosKernel");
+        countsFromPosKernel<<<ceilDiv(size_t(newN), 256), 256, 0, stream(s)>>>(posWs, newN, maxCount, newCounts);
+        HC_CHECK_LAUNCH("countsFromPosKernel");
+    }
+    else if (auto e = computeNodeCounts<K>(s, newTree, newCounts, newN, keys, numKeys, maxCount)) return e;
     *numNodes = newN;
     return HIPCSTONE_OK;
 }
+
+template<class K>
+hipcstone_status updateOctreeWarm(hipcstone_stream s, const K* keys, size_t numKeys, uint32_t bucketSize,
+                                  const K* tree, const uint32_t* counts, int32_t* numNodes, K* newTree,
+                                  uint32_t* newCounts, int32_t capacity, void* ws, size_t wsBytes, int* converged,
+                                  const uint32_t* oldPos, uint32_t* posWs)
+{
+    return updateOctree<K>(s, keys, numKeys, bucketSize, tree, counts, numNodes, newTree, newCounts, capacity,
+                           0xFFFFFFFFu, ws, wsBytes, converged, oldPos, posWs);
+}
+
+template hipcstone_status updateOctreeWarm<uint32_t>(hipcstone_stream, const uint32_t*, size_t, uint32_t,
+                                                     const uint32_t*, const uint32_t*, int32_t*, uint32_t*, uint32_t*,
+                                                     int32_t, void*, size_t, int*, const uint32_t*, uint32_t*);
+template hipcstone_status updateOctreeWarm<uint64_t>(hipcstone_stream, const uint64_t*, size_t, uint32_t,
+                                                     const uint64_t*, const uint32_t*, int32_t*, uint64_t*, uint32_t*,
+                                                     int32_t, void*, size_t, int*, const uint32_t*, uint32_t*);
 
 } // namespace hipcstone
 

@@ -414,8 +414,8 @@ public:
     size_t deviceBytes() const override
     {
         size_t b = 0;
-        for (const DeviceBuf* d : {&keys_, &keyBuf_, &perm_, &valBuf_, &sortWs_, &segWs_, &hintWs_, &updWs_, &splitKeys_,
-                                   &splitCounts_, &splitCounters_, &splitSortWs_, &prefixes_, &childOffsets_,
+        for (const DeviceBuf* d : {&keys_, &keyBuf_, &perm_, &valBuf_, &sortWs_, &segWs_, &hintWs_, &updWs_, &posWs_,
+                                   &splitKeys_, &splitCounts_, &splitCounters_, &splitSortWs_, &prefixes_, &childOffsets_,
                                    &parents_, &levelRange_, &internalToLeaf_, &leafToInternal_, &buildWs_, &centers_,
                                    &sizes_, &layout_, &scanWs_, &assignOff_, &assignBnd_, &assignCnt_, &assignWs_,
                                    &flags_, &searchC_, &searchS_, &localLayout_, &haloWs_, &groupsWs_})
@@ -455,6 +455,7 @@ private:
             return e;
         maxPending_    = defer;
         lastMaxBucket_ = defer ? ~0u : largest;
+        warmPos_       = done; // the bucket offsets are the old boundaries' positions for the tree update
         if (done) return HIPCSTONE_OK;
         return A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xo, yo, zo, ho, keyBuf_.as<K>(),
                              valBuf_.as<uint32_t>(), sortWs_.ptr, sortWs_.bytes);
@@ -512,14 +513,29 @@ private:
     // one updateOctree step from (tree_[treeCur_], counts_[treeCur_]); grows buffers and retries on CAPACITY
     hipcstone_status updateOnce(hipcstone_stream s, int* converged)
     {
+        // after a bucket sort through the same leaves, the node counts come from its bucket offsets
+        const bool warm = warmPos_;
+        warmPos_        = false;
         for (int attempt = 0; attempt < 8; ++attempt)
         {
-            int32_t nn  = numLeaves_;
-            int other   = 1 - treeCur_;
-            auto e      = A::update(s, keys_.as<K>(), n_, bucket_, tree_[treeCur_].template as<K>(),
-                                    counts_[treeCur_].template as<uint32_t>(), &nn, tree_[other].template as<K>(),
-                                    counts_[other].template as<uint32_t>(), capLeaves_, updWs_.ptr, updWs_.bytes,
-                                    converged);
+            int32_t nn = numLeaves_;
+            int other  = 1 - treeCur_;
+            hipcstone_status e;
+            if (warm)
+            {
+                if (auto g = posWs_.reserve((size_t(capLeaves_) + 1) * sizeof(uint32_t))) return g;
+                e = updateOctreeWarm<K>(s, keys_.as<K>(), n_, bucket_, tree_[treeCur_].template as<K>(),
+                                        counts_[treeCur_].template as<uint32_t>(), &nn, tree_[other].template as<K>(),
+                                        counts_[other].template as<uint32_t>(), capLeaves_, updWs_.ptr, updWs_.bytes,
+                                        converged, segWs_.as<uint32_t>() + numLeaves_, posWs_.as<uint32_t>());
+            }
+            else
+            {
+                e = A::update(s, keys_.as<K>(), n_, bucket_, tree_[treeCur_].template as<K>(),
+                              counts_[treeCur_].template as<uint32_t>(), &nn, tree_[other].template as<K>(),
+                              counts_[other].template as<uint32_t>(), capLeaves_, updWs_.ptr, updWs_.bytes,
+                              converged);
+            }
             if (e == HIPCSTONE_CAPACITY)
             {
                 if (auto g = reserveLeaves(nn)) return g;
@@ -537,6 +553,7 @@ private:
     hipcstone_status computeOctree(hipcstone_stream s)
     {
         numLeaves_ = 0; // nothing to keep across buffer growth
+        warmPos_   = false;
         if (auto e = reserveLeaves(std::max<int32_t>(capLeaves_, 4096))) return e;
         if (auto e = splitCounters_.reserve(2 * sizeof(uint32_t))) return e;
         if (auto e = frontier_[0].reserve(sizeof(SplitNode<K>))) return e;
@@ -666,6 +683,7 @@ private:
     int treeCur_       = 0;
     int convergedCalls_ = 0;
     uint32_t lastMaxBucket_     = ~0u;   // largest bucket of the last warm-step sort, ~0 when unknown
+    bool warmPos_               = false; // this step's bucket sort left the old boundaries' positions in segWs_
     bool maxPending_            = false; // this step's largest bucket is on its way to pinned word 9
     int32_t* levelRangePinned_  = nullptr; // hipHostMalloc'd
     hipEvent_t levelRangeReady_ = nullptr;
@@ -675,7 +693,7 @@ private:
 
     DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, segWs_, hintWs_;
     DeviceBuf xyzh_[2][4];
-    DeviceBuf tree_[2], counts_[2], updWs_;
+    DeviceBuf tree_[2], counts_[2], updWs_, posWs_;
     DeviceBuf frontier_[2], splitKeys_, splitCounts_, splitCounters_, splitSortWs_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_, groupsWs_;

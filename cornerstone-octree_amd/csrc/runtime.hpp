@@ -46,6 +46,15 @@ uint32_t sortLookbackLimit();
 // waves per workgroup of the neighbor search (hipcstone_set_neighbor_waves): 2, 4 or 8
 int neighborWaves();
 
+// updateOctree (maxCount unlimited) whose node counts come from oldPos[i] = lower_bound(keys, tree[i]), i in
+// [0, *numNodes] (the bucket offsets of this step's sort through the same leaves): a bounded search per split child
+// instead of one over all keys per node. posWs holds capacity + 1 words.
+template<class K>
+hipcstone_status updateOctreeWarm(hipcstone_stream s, const K* keys, size_t numKeys, uint32_t bucketSize,
+                                  const K* tree, const uint32_t* counts, int32_t* numNodes, K* newTree,
+                                  uint32_t* newCounts, int32_t capacity, void* ws, size_t wsBytes, int* converged,
+                                  const uint32_t* oldPos, uint32_t* posWs);
+
 // The neighbor search of hipcstone_find_neighbors_* with a workspace for the compact child blocks the fp32 open-box
 // walk reads (neighborGroupsBytes(numNodes) bytes; with less, or none, they are allocated on the stream per call).
 size_t neighborGroupsBytes(int32_t numNodes);
