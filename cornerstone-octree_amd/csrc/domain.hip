@@ -713,6 +713,7 @@ private:
                                                  valBuf_.as<uint32_t>(), bsWs_, bsScanWs_, &bucketed))
                 return e;
         }
+        warmPos_ = bucketed; // the bucket offsets are the old global boundaries' positions for the first update
         if (bucketed) return HIPCSTONE_OK;
         return A::sortGather(s, keys_.as<K>(), perm_.as<uint32_t>(), n, x, y, z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(),
                              xyzh_[2].as<T>(), xyzh_[3].as<T>(), keyBuf_.as<K>(), valBuf_.as<uint32_t>(),
@@ -771,15 +772,31 @@ private:
     // leaf count of the new tree
     hipcstone_status globalStep(hipcstone_stream s, uint32_t* result)
     {
+        // the first update after a bucket sort through the same global leaves takes its node counts from the
+        // bucket offsets
+        const bool warm = warmPos_;
+        warmPos_        = false;
         for (int attempt = 0; attempt < 8; ++attempt)
         {
             int32_t nn = numGlobal_;
             int other  = 1 - gCur_;
             int conv   = 0;
-            auto e     = A::update(s, keys_.as<K>(), n_, bucket_, gTree_[gCur_].template as<K>(),
-                                   gCounts_[gCur_].template as<uint32_t>(), &nn, gTree_[other].template as<K>(),
-                                   gCounts_[other].template as<uint32_t>(), capGlobal_, updWs_.ptr, updWs_.bytes,
-                                   &conv);
+            hipcstone_status e;
+            if (warm)
+            {
+                if (auto g = posWs_.reserve((size_t(capGlobal_) + 1) * sizeof(uint32_t))) return g;
+                e = updateOctreeWarm<K>(s, keys_.as<K>(), n_, bucket_, gTree_[gCur_].template as<K>(),
+                                        gCounts_[gCur_].template as<uint32_t>(), &nn, gTree_[other].template as<K>(),
+                                        gCounts_[other].template as<uint32_t>(), capGlobal_, updWs_.ptr,
+                                        updWs_.bytes, &conv, bsWs_.as<uint32_t>() + numGlobal_,
+                                        posWs_.as<uint32_t>());
+            }
+            else
+            {
+                e = A::update(s, keys_.as<K>(), n_, bucket_, gTree_[gCur_].template as<K>(),
+                              gCounts_[gCur_].template as<uint32_t>(), &nn, gTree_[other].template as<K>(),
+                              gCounts_[other].template as<uint32_t>(), capGlobal_, updWs_.ptr, updWs_.bytes, &conv);
+            }
             if (e == HIPCSTONE_CAPACITY)
             {
                 if (auto g = reserveGlobal(nn)) return g;
@@ -1014,7 +1031,8 @@ private:
 
     DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, mmPartials_, mmOut_, small_, bsWs_, bsScanWs_;
     DeviceBuf xyzh_[4];
-    DeviceBuf gTree_[2], gCounts_[2], updWs_;
+    DeviceBuf gTree_[2], gCounts_[2], updWs_, posWs_;
+    bool warmPos_ = false; // this call's bucket sort left the old global boundaries' positions in bsWs_
     DeviceBuf fLeaves_, fLeavesNew_, fLeafCounts_, fNodeCounts_, leafOps_, ops_, prot_;
     DeviceBuf prefixes_, childOffsets_, parents_, levelRange_, internalToLeaf_, leafToInternal_, buildWs_;
     DeviceBuf centers_, sizes_, layout_, scanWs_;
