@@ -217,17 +217,18 @@ __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ 
             }
         }
     }
-    // one atomic per distinct leaf of the wave: the first lane of each group adds the group's size
-    uint64_t todo   = ballot(valid);
-    uint32_t myBase = 0; // the atomic's result, in the leader lane of each group
-    uint32_t myRank = 0; // rank among the group's lanes
-    int myLeader    = 0;
+    // the wave's lanes grouped by leaf: the first lane of each group counts it, then one atomic instruction, issued
+    // by the first lanes of all groups together, adds every group's size (one round trip per wave, not per group)
+    uint64_t todo    = ballot(valid);
+    uint32_t myCount = 0; // the group's size, in its first lane
+    uint32_t myRank  = 0; // rank among the group's lanes
+    int myLeader     = 0;
     while (todo)
     {
         const int leader     = __ffsll((unsigned long long)todo) - 1;
         const uint32_t g     = uint32_t(__builtin_amdgcn_readlane(int(lo), leader));
         const uint64_t peers = ballot(valid && lo == g);
-        if (lane == leader) myBase = atomicAdd(&segCount[g], uint32_t(__popcll(peers)));
+        if (lane == leader) myCount = uint32_t(__popcll(peers));
         if (laneIn(peers))
         {
             myLeader = leader;
@@ -235,12 +236,11 @@ __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ 
         }
         todo &= ~peers;
     }
+    if (valid) bucket[i] = lo;
+    uint32_t myBase = 0; // the atomic's result, in the first lane of each group
+    if (myCount != 0) myBase = atomicAdd(&segCount[lo], myCount);
     const uint32_t base = uint32_t(__shfl(int(myBase), myLeader, kWave));
-    if (valid)
-    {
-        bucket[i] = lo;
-        slot[i]   = base + myRank;
-    }
+    if (valid) slot[i] = base + myRank;
 }
 
 // largest bucket: one atomicMax per block into *out (zeroed by the caller)
@@ -340,11 +340,19 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
         zo[pos]      = z[id];
         ho[pos]      = h[id];
     };
-    // the window at p: each lane's element, its bucket [r.x, r.y) and input index
-    uint2 r;
-    K k;
-    uint32_t id;
-    auto load = [&](uint32_t p) {
+    // A window starts at a bucket boundary P: each lane's element P + lane, its bucket [r.x, r.y) and input index.
+    // The buckets come from the offsets alone, so the wave knows where every window starts before any key arrives,
+    // and runs two windows ahead: the keys and indices of window 2 and the x,y,z,h of window 1 are in flight while
+    // window 0 is ranked and stored.
+    struct Window
+    {
+        uint32_t P;
+        uint2 r;
+        K k;
+        uint32_t id;
+    };
+    auto open = [&](Window& W, uint32_t p) {
+        W.P              = p;
         const uint32_t e = p + uint32_t(lane);
         const bool valid = e < end;
         // the bucket [off[j], off[j + 1]) holding e: the last offset at or below e and the first above it (the
@@ -356,19 +364,56 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
             rx = off[j] <= e ? off[j] : rx;
             ry = off[j] > e && off[j] < ry ? off[j] : ry;
         }
-        r  = valid ? make_uint2(rx, ry) : make_uint2(0u, 0u);
-        k  = valid ? keysIn[e] : K(0);
-        id = valid ? idxIn[e] : 0u;
+        W.r  = valid ? make_uint2(rx, ry) : make_uint2(0u, 0u);
+        W.k  = valid ? keysIn[e] : K(0);
+        W.id = valid ? idxIn[e] : 0u;
     };
-    uint32_t P = uniform(segOffset[b0]);
-    if (P < end) load(P);
-    while (P < end)
+    // the end of lane 0's bucket: a window whose first bucket holds more than 64 elements is that bucket alone
+    auto firstEnd = [&](const Window& W) -> uint32_t { return uniform(W.r.y); };
+    auto isBig    = [&](const Window& W) -> bool { return firstEnd(W) - W.P > uint32_t(kWave); };
+    // lanes of whole buckets: a prefix of the lanes (lane 0's bucket fits unless the window is big)
+    auto wholeLane = [&](const Window& W) -> bool { return W.r.y != 0u && W.r.y <= W.P + uint32_t(kWave); };
+    // the start of the window after W (end when W is the last)
+    auto after = [&](const Window& W) -> uint32_t {
+        if (W.P >= end) return end;
+        if (isBig(W)) return firstEnd(W);
+        const int lastWhole = __popcll(ballot(wholeLane(W))) - 1;
+        return uint32_t(__builtin_amdgcn_readlane(int(W.r.y), lastWhole));
+    };
+    struct Props
     {
-        const uint32_t firstEnd = uniform(r.y); // lane 0 holds element P, the first of its bucket
-        if (firstEnd - P > uint32_t(kWave))
+        T x, y, z, h;
+    };
+    auto gather = [&](const Window& W) -> Props {
+        Props v{T(0), T(0), T(0), T(0)};
+        if (W.P < end && !isBig(W) && wholeLane(W))
+        {
+            v.x = x[W.id];
+            v.y = y[W.id];
+            v.z = z[W.id];
+            v.h = h[W.id];
+        }
+        return v;
+    };
+
+    Window W0{}, W1{}, W2{};
+    if (off[0] >= end) return;
+    open(W0, off[0]);
+    const uint32_t p1 = after(W0);
+    if (p1 < end) open(W1, p1);
+    else W1.P = end;
+    Props v0 = gather(W0);
+    while (W0.P < end)
+    {
+        const uint32_t p2 = after(W1);
+        if (p2 < end) open(W2, p2);
+        else W2.P = end;
+        const Props v1 = gather(W1);
+        const uint32_t P = W0.P;
+        if (isBig(W0))
         {
             // a bucket of more than 64 elements, [P, firstEnd)
-            const uint32_t cnt = firstEnd - P;
+            const uint32_t cnt = firstEnd(W0) - P;
             for (uint32_t c0 = 0; c0 < cnt; c0 += kWave)
             {
                 const bool mine    = c0 + uint32_t(lane) < cnt;
@@ -395,50 +440,46 @@ __global__ __launch_bounds__(256) void sortBucketsGatherKernel(const K* __restri
                 }
                 if (mine) put(P + rank, kc, idc);
             }
-            P = firstEnd;
-            if (P < end) load(P);
-            continue;
         }
-        // whole buckets of the window: a prefix of its lanes (buckets are contiguous); the next window starts after
-        // the last of them
-        const bool whole    = r.y != 0u && r.y <= P + uint32_t(kWave);
-        const int lastWhole = __popcll(ballot(whole)) - 1; // >= 0: lane 0's bucket fits
-        const uint32_t Pn   = uint32_t(__builtin_amdgcn_readlane(int(r.y), lastWhole));
-        // this window's x,y,z,h and the next window's elements are loaded while the ranks are counted
-        T xv = T(0), yv = T(0), zv = T(0), hv = T(0);
-        if (whole)
+        else
         {
-            xv = x[id];
-            yv = y[id];
-            zv = z[id];
-            hv = h[id];
-        }
-        const uint2 rc     = r;
-        const K kc         = k;
-        const uint32_t idc = id;
-        if (Pn < end) load(Pn);
-        waveSync();
-        wKey[lane] = kc;
-        wIdx[lane] = idc;
-        waveSync();
-        if (whole)
-        {
-            uint32_t rank = 0;
-            for (uint32_t q = rc.x - P; q < rc.y - P; ++q)
+            const bool whole = wholeLane(W0);
+            waveSync();
+            wKey[lane] = W0.k;
+            wIdx[lane] = W0.id;
+            waveSync();
+            // the whole buckets' keys ascend from bucket to bucket, so when every whole lane follows its left
+            // neighbor in (key, index) order, lane order is the sorted order and the ranking is skipped: the common
+            // case, since the keys arrive in the previous step's order and a bucket's slots follow lane order within
+            // each wave
+            const int left         = lane > 0 ? lane - 1 : 0;
+            const uint32_t follows = lane > 0 ? addIfPairLess(0u, wKey[left], wIdx[left], W0.k, W0.id) : 1u;
+            const bool inOrder     = ballot(whole && follows == 0u) == 0;
+            if (whole)
             {
-                const K kq        = wKey[q];
-                const uint32_t iq = wIdx[q];
-                rank              = addIfPairLess(rank, kq, iq, kc, idc);
+                uint32_t pos = P + uint32_t(lane);
+                if (!inOrder)
+                {
+                    uint32_t rank = 0;
+                    for (uint32_t q = W0.r.x - P; q < W0.r.y - P; ++q)
+                    {
+                        const K kq        = wKey[q];
+                        const uint32_t iq = wIdx[q];
+                        rank              = addIfPairLess(rank, kq, iq, W0.k, W0.id);
+                    }
+                    pos = W0.r.x + rank;
+                }
+                keysOut[pos] = W0.k;
+                permOut[pos] = W0.id;
+                xo[pos]      = v0.x;
+                yo[pos]      = v0.y;
+                zo[pos]      = v0.z;
+                ho[pos]      = v0.h;
             }
-            const uint32_t pos = rc.x + rank;
-            keysOut[pos]       = kc;
-            permOut[pos]       = idc;
-            xo[pos]            = xv;
-            yo[pos]            = yv;
-            zo[pos]            = zv;
-            ho[pos]            = hv;
         }
-        P = Pn;
+        W0 = W1;
+        W1 = W2;
+        v0 = v1;
     }
 }
 
