@@ -243,15 +243,28 @@ __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ 
     if (valid) slot[i] = base + myRank;
 }
 
-// largest bucket: one atomicMax per block into *out (zeroed by the caller)
+// largest bucket: each block reduces kMaxPerBlock counts, whose loads are all in flight together, and makes at most
+// one atomicMax into *out (zeroed by the caller); few blocks keep the atomics on that one word few
+constexpr uint32_t kMaxPerThread = 16;
+constexpr uint32_t kMaxPerBlock  = 256u * kMaxPerThread;
+
 template<int Unused = 0>
 __global__ __launch_bounds__(256) void maxBucketKernel(const uint32_t* __restrict__ counts, uint32_t n,
                                                        uint32_t* __restrict__ out)
 {
     __shared__ uint32_t sh[4];
+    const uint64_t base = uint64_t(blockIdx.x) * kMaxPerBlock + threadIdx.x;
+    uint32_t v[kMaxPerThread];
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxPerThread; ++k)
+    {
+        const uint64_t i = base + uint64_t(k) * 256u;
+        v[k]             = i < n ? counts[i] : 0u;
+    }
     uint32_t m = 0;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
-        m = counts[i] > m ? counts[i] : m;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxPerThread; ++k)
+        m = v[k] > m ? v[k] : m;
     for (int off = 32; off >= 1; off >>= 1)
     {
         const uint32_t o = uint32_t(__shfl_xor(int(m), off, kWave));
@@ -540,7 +553,7 @@ hipcstone_status sortByLeafBuckets(hipcstone_stream s, K* keys, size_t n, const 
     }
     bucketByLeafKernel<K><<<ceilDiv(n, 256), 256, 0, st>>>(keys, n32, leaves, L, segCount, bucketBuf, slot, hint);
     HC_CHECK_LAUNCH("bucketByLeafKernel");
-    maxBucketKernel<<<std::min(ceilDiv(size_t(L), 256), 1024u), 256, 0, st>>>(segCount, uint32_t(L), maxBucket);
+    maxBucketKernel<<<ceilDiv(size_t(L), kMaxPerBlock), 256, 0, st>>>(segCount, uint32_t(L), maxBucket);
     HC_CHECK_LAUNCH("maxBucketKernel");
     if (maxOut && deferMax) { HC_CHECK(hipMemcpyAsync(maxOut, maxBucket, sizeof(uint32_t), hipMemcpyDeviceToHost, st)); }
     else
