@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void bucketByLeafKernel(const K* __restrict__ 
     if (hint)
     {
         const uint32_t l0 = uniform(hint[uniform(i / 64u)]);
-        if (l0 + kHintLeaves <= uint32_t(L))
+        if (uint32_t(L) >= kHintLeaves && l0 <= uint32_t(L) - kHintLeaves) // no wrap for any hint value
         {
             // the hint's kHintLeaves + 1 boundaries with wave-uniform addresses: a few wide scalar loads issued
             // together instead of one dependent load per boundary

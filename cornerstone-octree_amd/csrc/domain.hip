@@ -505,7 +505,8 @@ public:
         T* z = static_cast<T*>(zv);
         T* h = static_cast<T*>(hv);
         if (auto e = reserveParticles(n)) return e;
-        n_ = n;
+        const size_t prevN = n_;
+        n_                 = n;
         auto& tm = this->timer;
 #define HC_STAGE(id, call)                                                                                             \
     do {                                                                                                               \
@@ -515,7 +516,7 @@ public:
     } while (0)
         HC_STAGE(0, updateBox(s, x, y, z));
         HC_STAGE(1, A::keys(s, x, y, z, keys_.as<K>(), n, &box_));
-        HC_STAGE(2, sortAndGather(s, x, y, z, h, n));
+        HC_STAGE(2, sortAndGather(s, x, y, z, h, n, prevN));
         HC_STAGE(3, globalUpdate(s));
         if (first_)
         {
@@ -702,16 +703,47 @@ private:
     // stable sort + gather of the call's particles into xyzh_ (one timed stage): later calls go through the buckets
     // of the previous global leaf array, which hands a step with a bucket above its limit to the radix sort (the
     // same result either way)
-    hipcstone_status sortAndGather(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, size_t n)
+    // prevN: the particle count of the previous sync
+    hipcstone_status sortAndGather(hipcstone_stream s, const T* x, const T* y, const T* z, const T* h, size_t n,
+                                   size_t prevN)
     {
         bool bucketed = false;
+        uint32_t* pin = pinnedStaging();
+        if (!pin) return HIPCSTONE_HIP_ERROR;
+        if (maxPending_)
+        {
+            lastMaxBucket_ = pin[kMaxWord]; // the previous sort's largest bucket, copied before its syncs
+            maxPending_    = false;
+        }
         if (numGlobal_ > 0)
         {
+            // the global leaves' counts are exact (no cap), so with an unchanged particle count their layout is the
+            // previous step's, over the previous positions of the particles the keys come in: a search hint per wave
+            // (any input order sorts the same)
+            const uint32_t* prevLayout = nullptr;
+            if (n == prevN)
+            {
+                if (auto e = gLayout_.reserve((size_t(numGlobal_) + 1) * sizeof(uint32_t))) return e;
+                if (auto e = bsScanWs_.reserve(hipcstone_scan_workspace_size(size_t(numGlobal_)))) return e;
+                if (auto e = hipcstone_leaf_layout(s, gCounts_[gCur_].template as<uint32_t>(), numGlobal_,
+                                                   gLayout_.as<uint32_t>(), bsScanWs_.ptr, bsScanWs_.bytes))
+                    return e;
+                prevLayout = gLayout_.as<uint32_t>();
+            }
+            // when the previous sort's largest bucket was well below the guard, this one's is not read first: it
+            // reaches pinned word kMaxWord with the global update's synchronization and guards the next sort
+            const bool defer = lastMaxBucket_ <= kMaxBucket / 4;
+            uint32_t largest = ~0u;
+            uint32_t* maxOut = defer ? pin + kMaxWord : &largest;
+            *maxOut          = ~0u;
             if (auto e = sortByLeafBuckets<K, T>(s, keys_.as<K>(), n, gTree_[gCur_].template as<K>(), numGlobal_, x, y,
                                                  z, h, xyzh_[0].as<T>(), xyzh_[1].as<T>(), xyzh_[2].as<T>(),
                                                  xyzh_[3].as<T>(), perm_.as<uint32_t>(), keyBuf_.as<K>(),
-                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_, &bucketed))
+                                                 valBuf_.as<uint32_t>(), bsWs_, bsScanWs_, &bucketed, prevLayout,
+                                                 &hintWs_, maxOut, defer))
                 return e;
+            maxPending_    = defer;
+            lastMaxBucket_ = defer ? ~0u : largest;
         }
         warmPos_ = bucketed; // the bucket offsets are the old global boundaries' positions for the first update
         if (bucketed) return HIPCSTONE_OK;
@@ -1029,7 +1061,12 @@ private:
     int32_t fL_ = 0, fN_ = 0;
     int globalSteps_ = 0, focusSteps_ = 0, focusConverged_ = 0;
 
-    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, mmPartials_, mmOut_, small_, bsWs_, bsScanWs_;
+    DeviceBuf keys_, keyBuf_, perm_, valBuf_, sortWs_, mmPartials_, mmOut_, small_, bsWs_, bsScanWs_, gLayout_,
+        hintWs_;
+    // pinned word of the calling thread that receives a deferred largest bucket (the pipeline's is word 9)
+    static constexpr int kMaxWord = 10;
+    uint32_t lastMaxBucket_ = ~0u;  // largest bucket of the last bucket sort, ~0 when unknown
+    bool maxPending_        = false; // the last sort's largest bucket is on its way to pinned word kMaxWord
     DeviceBuf xyzh_[4];
     DeviceBuf gTree_[2], gCounts_[2], updWs_, posWs_;
     bool warmPos_ = false; // this call's bucket sort left the old global boundaries' positions in bsWs_
