@@ -2,7 +2,7 @@
 
 The reference fixes its tree parameters per call (bucketSize) and its GPU traversal tiling at compile time
 (traversal/find_neighbors.cuh:29-58 TravConfig). Here the bucket size of every cloud of a batch is picked from
-{16, 32, 64, 128} by timing: clouds are grouped into classes (distribution kind x size decade); for each class a
+{16, 32, 64, 128, 256} by timing: clouds are grouped into classes (distribution kind x size decade); for each class a
 representative batch of clouds of that class is built and stepped (drift + keys + sort + one warm update + link +
 centers + layout + neighbors, captured in a HIP graph) at every candidate bucket size, and the class takes the bucket
 size with the least time per particle. The choices are persisted in tuned.json next to this file, which the batched
@@ -24,7 +24,7 @@ import time
 
 import numpy as np
 
-CANDIDATES = (16, 32, 64, 128)
+CANDIDATES = (16, 32, 64, 128, 256)
 KINDS = ("uniform", "plummer")
 # size bands of the classes: [1e3, 1e4), [1e4, 1e5), [1e5, 1e6]
 BANDS = ((1_000, 10_000), (10_000, 100_000), (100_000, 1_000_001))

@@ -14,7 +14,7 @@ from oracle_py import make_box as orc_box
 
 pytestmark = pytest.mark.gpu
 
-BUCKETS = (16, 32, 64, 128)
+BUCKETS = (16, 32, 64, 128, 256)
 
 
 def _mix(seed, num, nmin, nmax, target=40.0):
@@ -78,7 +78,7 @@ def _compare(b, plain, tag):
 def test_batched_equals_plain_per_cloud(hip, oracle, curve_name):
     curve = hip.MORTON if curve_name == "morton" else hip.HILBERT
     sizes, boxes, kinds, x, y, z, h = _mix(seed=77 if curve == hip.MORTON else 78, num=24, nmin=300, nmax=30000)
-    buckets = np.array([BUCKETS[c % 4] for c in range(len(sizes))], dtype=np.uint32)
+    buckets = np.array([BUCKETS[c % len(BUCKETS)] for c in range(len(sizes))], dtype=np.uint32)
     ngmax = 48
     print(f"batched vs plain, {curve_name}: {len(sizes)} clouds, {int(sizes.sum())} particles", flush=True)
     b = hip.Batched(sizes, [hip.make_box(lo, hi, np.float32) for lo, hi in boxes], buckets, curve=curve, ngmax=ngmax)
@@ -118,7 +118,7 @@ def test_batched_equals_plain_per_cloud(hip, oracle, curve_name):
 
 def test_batched_graph_replays_equal_plain_steps(hip):
     sizes, boxes, kinds, x, y, z, h = _mix(seed=79, num=17, nmin=1000, nmax=20000)
-    buckets = np.array([BUCKETS[(3 * c + 1) % 4] for c in range(len(sizes))], dtype=np.uint32)
+    buckets = np.array([BUCKETS[(3 * c + 1) % len(BUCKETS)] for c in range(len(sizes))], dtype=np.uint32)
     ngmax = 40
     print(f"batched graph: {len(sizes)} clouds, {int(sizes.sum())} particles", flush=True)
     s = hip.Stream()
